@@ -1,0 +1,83 @@
+"""Synthetic circuit recipes used by the benchmark and the parity tests.
+
+Gate draws come from numpy's PCG64 generator seeded with the given seed, so
+every consumer (GPU path, CPU oracle, golden fixtures) sees the same gates.
+A circuit is a list of ``(name, qbits, params)`` with q1tsim's qubit
+numbering (qubit 0 = most significant index bit).
+"""
+import math
+
+import numpy as np
+
+ONE_QUBIT = ("h", "x", "s", "t", "rx", "ry", "rz")
+ROTATIONS = ("rx", "ry", "rz")
+
+
+def random_layered(nr_qbits, depth, seed):
+    """Layered random circuit: per layer one gate per qubit drawn uniformly
+    from {H, X, S, T, RX, RY, RZ} (theta ~ U[0, 2pi)), then CX on nearest
+    neighbours (0,1),(2,3),... on even layers and (1,2),(3,4),... on odd ones.
+    n=18, depth 20 gives 360 one-qubit gates + 170 CX; n=30 gives 600 + 290."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ops = []
+    for layer in range(depth):
+        for q in range(nr_qbits):
+            g = ONE_QUBIT[int(rng.integers(len(ONE_QUBIT)))]
+            if g in ROTATIONS:
+                ops.append((g, [q], [float(rng.uniform(0.0, 2.0 * math.pi))]))
+            else:
+                ops.append((g, [q], []))
+        for q in range(layer % 2, nr_qbits - 1, 2):
+            ops.append(("cx", [q, q + 1], []))
+    return ops
+
+
+def random_pairs(nr_qbits, depth, seed, two_qubit_frac=0.25):
+    """Depth-`depth` circuit whose slots are two-qubit CX on random (not only
+    neighbouring) pairs with probability `two_qubit_frac`, else a random
+    one-qubit gate from the layered recipe's set."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ops = []
+    for _ in range(depth):
+        free = list(range(nr_qbits))
+        rng.shuffle(free)
+        while free:
+            q = free.pop()
+            if free and rng.uniform() < two_qubit_frac:
+                r = free.pop()
+                ops.append(("cx", [q, r], []))
+            else:
+                g = ONE_QUBIT[int(rng.integers(len(ONE_QUBIT)))]
+                if g in ROTATIONS:
+                    ops.append((g, [q], [float(rng.uniform(0.0, 2.0 * math.pi))]))
+                else:
+                    ops.append((g, [q], []))
+    return ops
+
+
+def qft(nr_qbits):
+    """Quantum Fourier transform: H on qubit j, then CU1(2pi/2^(k-j+1))
+    controlled by qubit k > j, then the swap network (j, n-1-j).  With
+    qubit 0 the most significant bit this maps a -> sqrt(N) * ifft(a)."""
+    ops = []
+    for j in range(nr_qbits):
+        ops.append(("h", [j], []))
+        for k in range(j + 1, nr_qbits):
+            ops.append(("cu1", [k, j], [2.0 * math.pi / (1 << (k - j + 1))]))
+    for j in range(nr_qbits // 2):
+        ops.append(("swap", [j, nr_qbits - 1 - j], []))
+    return ops
+
+
+def unfused_bytes(ops, state_bytes):
+    """Minimal traffic of each gate run on its own: 2S for dense one-qubit
+    gates, RZ and X; S for S, T, CX and Swap; S/2 for CU1 (S = state bytes)."""
+    total = 0.0
+    for name, _, _ in ops:
+        if name in ("s", "t", "sdg", "tdg", "z", "cx", "swap", "cz"):
+            total += state_bytes
+        elif name in ("cu1", "cs", "ct", "csdg", "ctdg"):
+            total += 0.5 * state_bytes
+        else:
+            total += 2.0 * state_bytes
+    return total
