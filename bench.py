@@ -1,0 +1,192 @@
+"""Benchmark: 30-qubit random circuit on one MI355X.
+
+Workload (the headline configuration): the 30-qubit depth-20 layered random
+circuit (600 one-qubit gates from {H,X,S,T,RX,RY,RZ} + 290 nearest-neighbour
+CX), statevector resident in HBM.  A step is one application of the whole
+circuit to the resident state.
+
+  * plain  : complex128 state, per-gate FP64 arithmetic, gates grouped into
+             LDS/register tile sweeps (the reported `value`, in HBM GB/s)
+  * fused  : complex64 state with the fused-block plan; its time, speedup
+             over plain and infidelity / max-abs error against the plain
+             complex128 state are reported alongside.
+
+Also reported: the 30-qubit QFT on a random normal state and FP64 sampling of
+2^16 runs (measure_all, peek semantics).  `roofline` is the dominant kernel's
+algorithmic bytes per launch over its average HIP-event duration on the
+state's stream; `cpu_baseline` times the CPU restatement of the reference
+(the oracle, one core) on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "30-qubit circuit time: fused c64 vs plain c128 at infidelity<1e-6; HBM GB/s"
+PEAK_GBPS = 8000.0
+
+
+def apply_circuit(state, ops):
+    from q1tsim_amd.gvs import gate_matrix
+    for name, bits, params in ops:
+        state.apply_gate(gate_matrix(name, params), bits, name)
+    state.flush()
+
+
+def timed_steps(state, ops, steps, warmup, profile=True):
+    for _ in range(warmup):
+        apply_circuit(state, ops)
+    state.set_profiling(profile)
+    times, sweeps, nbytes = [], 0, 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        apply_circuit(state, ops)
+        times.append(time.perf_counter() - t0)
+        st = state.stats()
+        sweeps, nbytes = st["sweeps"], st["bytes"]
+    prof = state.kernel_profile()
+    state.set_profiling(False)
+    return times, sweeps, nbytes, prof
+
+
+def roofline_of(prof):
+    name = max(prof, key=lambda k: prof[k][1])
+    launches, ms, nbytes = prof[name]
+    avg_ms = ms / launches
+    achieved = nbytes / launches / (avg_ms * 1e-3) / 1e9
+    return name, {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                  "frac": round(achieved / PEAK_GBPS, 4), "traffic": None,
+                  "kernel": name, "avg_ms": round(avg_ms, 4), "bytes_per_launch": nbytes / launches,
+                  "launches": launches}
+
+
+def cpu_baseline(ops_full, n_full, budget_s=20.0):
+    """The oracle (C restatement of VectorState, 1 core) on the first gates of
+    the same circuit at n=26 (1 GiB state), extrapolated linearly in gates
+    and in 2^n to the full 30-qubit circuit."""
+    from oracle import oracle as orc
+    from q1tsim_amd import workloads
+    n = 26
+    ops = workloads.random_layered(n, 20, 1)
+    st = orc.RefState(n, 1)
+    t0 = time.perf_counter()
+    done = 0
+    for name, bits, params in ops:
+        st.apply_gate(name, bits, params)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    scale_n = float(1 << (n_full - n))
+    est_circuit_s = dt / done * len(ops_full) * scale_n
+    S = float(16 << n_full)
+    unf = workloads.unfused_bytes(ops_full, S)
+    return {"value": round(unf / est_circuit_s / 1e9, 3), "unit": "GB/s (unfused-equivalent)", "cores": 1,
+            "kind": "port",
+            "sample": "oracle (C restatement of q1tsim VectorState, FP64, 1 thread) applying the first %d gates "
+                      "of the layered recipe at n=%d in %.1f s; extrapolated x%d gates-ratio and x%d for 2^30 "
+                      "-> %.1f s per 30-qubit circuit" % (done, n, dt, len(ops_full) // max(done, 1), int(scale_n),
+                                                          est_circuit_s),
+            "circuit_seconds_est": round(est_circuit_s, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--fusion-k", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the QFT and sampling lines")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; multi-GPU sharding is not part of this build (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from q1tsim_amd import workloads
+    from q1tsim_amd.gvs import GpuVectorState, compare
+
+    n = args.n
+    ops = workloads.random_layered(n, args.depth, 1)
+    S128 = float(16 << n)
+
+    # plain complex128
+    plain = GpuVectorState(n, 1, "c128")
+    plain.set_plan(0, True)
+    t_plain, sw_plain, bytes_plain, prof_plain = timed_steps(plain, ops, args.steps, args.warmup)
+    ms_plain = 1e3 * float(np.median(t_plain))
+    kname, roof = roofline_of(prof_plain)
+
+    # fused complex64
+    fused = GpuVectorState(n, 1, "c64")
+    fused.set_plan(args.fusion_k, True)
+    t_fused, sw_fused, bytes_fused, prof_fused = timed_steps(fused, ops, args.steps, args.warmup)
+    ms_fused = 1e3 * float(np.median(t_fused))
+    infid, maxabs = compare(plain, fused)
+    del fused
+
+    extra = {}
+    if not args.no_extra:
+        # QFT on a random normal state + FP64 sampling of 2^16 runs
+        plain.set_random_normal(7)
+        q = workloads.qft(n)
+        t0 = time.perf_counter()
+        apply_circuit(plain, q)
+        extra["qft30_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
+        extra["qft30_sweeps"] = plain.stats()["sweeps"]
+        res = np.zeros(1 << 16, np.uint64)
+        from q1tsim_amd.gvs import Xoshiro
+        t0 = time.perf_counter()
+        plain.measure_all_into(list(range(n)), res, Xoshiro(1), collapse=False)
+        extra["sample65536_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
+
+    value = bytes_plain / (ms_plain * 1e-3) / 1e9
+    unf = workloads.unfused_bytes(ops, S128)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_plain, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "30-qubit depth-20 layered random circuit (600 1q + 290 CX), statevector in HBM; "
+                               "value = plain complex128 path",
+                   "n_qubits": n, "depth": args.depth, "gates": len(ops), "seed": 1},
+        "plain_c128": {"ms": round(ms_plain, 3), "sweeps": sw_plain, "hbm_bytes": bytes_plain,
+                       "GBps": round(value, 1), "frac_of_peak": round(value / PEAK_GBPS, 4),
+                       "unfused_equivalent_GBps": round(unf / (ms_plain * 1e-3) / 1e9, 1)},
+        "fused_c64": {"ms": round(ms_fused, 3), "sweeps": sw_fused, "hbm_bytes": bytes_fused,
+                      "GBps": round(bytes_fused / (ms_fused * 1e-3) / 1e9, 1), "fusion_k": args.fusion_k,
+                      "speedup_vs_plain": round(ms_plain / ms_fused, 3),
+                      "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs, "dtype": "f32"},
+        "roofline": roof,
+        "kernels_plain": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof_plain.items()},
+        "kernels_fused": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof_fused.items()},
+    }
+    out.update(extra)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ops, n)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

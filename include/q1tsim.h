@@ -1,0 +1,92 @@
+/*
+ * q1tsim MI355X statevector backend -- circuit-level C ABI.
+ *
+ * The symbols, struct layouts and result conventions of q1tsim's src/ffi.rs,
+ * so python/q1tsimffi.py's cdef (q1tsimffi.py:35-82) binds this library
+ * (libq1tsim.so) unchanged.  Every circuit executes on the GPU statevector
+ * (q1tsim_gvs.h); the q1t_circuit_* functions are build-added controls.
+ */
+#ifndef Q1TSIM_H
+#define Q1TSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "q1tsim_gvs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* result types (ffi.rs:5-9) */
+#define RESULT_ERROR 0
+#define RESULT_EMPTY 1
+#define RESULT_STRING 2
+#define RESULT_HISTOGRAM 3
+#define RESULT_CSTATE 5
+
+typedef struct circuit circuit_t;
+
+/* CHistElem (ffi.rs:25-30) */
+typedef struct {
+    const char* key;
+    size_t count;
+} histelem_t;
+
+/* CParameter (ffi.rs:40-46): a non-null value_ptr is read at execute time */
+typedef struct {
+    double value;
+    const double* value_ptr;
+} parameter_t;
+
+/* CResult (ffi.rs:63-70) */
+typedef struct {
+    void* data;
+    size_t length;
+    size_t size;
+    uint32_t restype;
+} result_t;
+
+void result_free(result_t res);                                       /* ffi.rs:165-169 */
+circuit_t* circuit_new(size_t nr_qbits, size_t nr_cbits);             /* ffi.rs:171-176 */
+void circuit_free(circuit_t* ptr);                                    /* ffi.rs:178-186 */
+size_t circuit_nr_qbits(const circuit_t* ptr);                        /* ffi.rs:188-194 */
+size_t circuit_nr_cbits(const circuit_t* ptr);                        /* ffi.rs:196-202 */
+result_t circuit_cstate(const circuit_t* ptr);                        /* ffi.rs:204-214 */
+result_t circuit_add_gate(circuit_t* ptr, const char* gate, const size_t* qbits, size_t nr_qbits,
+                          const parameter_t* params, size_t nr_params);  /* ffi.rs:307-380 */
+result_t circuit_add_conditional_gate(circuit_t* ptr, const size_t* control, size_t nr_control, uint64_t target,
+                                      const char* gate, const size_t* qbits, size_t nr_qbits,
+                                      const parameter_t* params, size_t nr_params);  /* ffi.rs:382-459 */
+result_t circuit_reset(circuit_t* ptr, size_t qbit);                  /* ffi.rs:461-477 */
+result_t circuit_reset_all(circuit_t* ptr);                           /* ffi.rs:479-492 */
+result_t circuit_measure(circuit_t* ptr, size_t qbit, size_t cbit, char dir, uint8_t collapse);  /* ffi.rs:494-522 */
+result_t circuit_measure_all(circuit_t* ptr, const size_t* cbits, size_t nr_cbits, char dir,
+                             uint8_t collapse);                       /* ffi.rs:524-557 */
+result_t circuit_execute(circuit_t* ptr, size_t nr_shots);            /* ffi.rs:559-575 */
+result_t circuit_reexecute(circuit_t* ptr);                           /* ffi.rs:577-593 */
+result_t circuit_histogram(const circuit_t* ptr);                     /* ffi.rs:595-611 */
+result_t circuit_latex(const circuit_t* ptr);                         /* ffi.rs:613-629 (export: not provided) */
+result_t circuit_open_qasm(const circuit_t* ptr);                     /* ffi.rs:632-648 (export: not provided) */
+result_t circuit_c_qasm(const circuit_t* ptr);                        /* ffi.rs:650-666 (export: not provided) */
+
+/* ---- build-added controls (no ffi.rs counterpart) ---- */
+/* precision Q1T_PREC_C128 (plain, FP64) or Q1T_PREC_C64; see q1t_gvs_set_plan */
+int q1t_circuit_set_backend(circuit_t* ptr, int precision, int fusion_k, int tiling);
+/* seed of the built-in xoshiro256** stream used by execute/reexecute */
+int q1t_circuit_seed(circuit_t* ptr, uint64_t seed);
+/* initial state for execute: 2^n interleaved amplitudes, or a seeded N(0,1) state */
+int q1t_circuit_set_initial_amplitudes(circuit_t* ptr, const double* amps, size_t len);
+int q1t_circuit_set_initial_random(circuit_t* ptr, uint64_t seed);
+int q1t_circuit_barrier(circuit_t* ptr, const size_t* qbits, size_t n);     /* Circuit::barrier (circuit.rs:543-554) */
+int q1t_circuit_nr_columns(circuit_t* ptr, size_t* out);
+int q1t_circuit_counts(circuit_t* ptr, size_t* out, size_t len);
+int q1t_circuit_read_amplitudes(circuit_t* ptr, size_t col, double* out, size_t len);
+int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out);
+/* histogram_vec (circuit.rs:800-815): counts of length 2^nr_cbits */
+int q1t_circuit_histogram_vec(const circuit_t* ptr, size_t* out, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,162 @@
+/*
+ * q1tsim MI355X statevector backend -- state-level C ABI.
+ *
+ * One opaque q1t_gvs is one GpuVectorState: the replacement for q1tsim's
+ * VectorState behind the QuState trait (src/qustate.rs:5-89).  A Rust
+ * `GpuVectorState` implements QuState by calling these functions (the
+ * binding is shown in INTEGRATION.md); gates arrive as their FP64
+ * Gate::matrix() plus qubit indices, exactly what the trait hands a backend.
+ *
+ * Conventions (identical to the reference):
+ *  - qubit q is index bit n-1-q (qubit 0 is the most significant bit);
+ *  - a k-qubit gate matrix is 2^k x 2^k, row-major, complex interleaved
+ *    (re, im) doubles, the first listed qubit being the most significant bit
+ *    of the local index (gates.rs:176-183);
+ *  - runs of column j are the contiguous range sum(counts[:j]) .. of the
+ *    result array (vectorstate.rs:283-297).
+ *
+ * All functions return 0 on success or a Q1T_ERR_* code; q1t_last_error()
+ * then returns the message (the Display text of the matching src/error.rs
+ * variant where one exists).  Gates are queued and applied lazily: any
+ * measurement, read-out or q1t_gvs_flush applies the queue as planned sweeps.
+ */
+#ifndef Q1TSIM_GVS_H
+#define Q1TSIM_GVS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    Q1T_OK = 0,
+    Q1T_ERR_INVALID_NR_BITS = 1,              /* Error::InvalidNrBits */
+    Q1T_ERR_INVALID_QBIT = 2,                 /* Error::InvalidQBit */
+    Q1T_ERR_INVALID_CBIT = 3,                 /* Error::InvalidCBit */
+    Q1T_ERR_INVALID_NR_CONTROL_BITS = 4,      /* Error::InvalidNrControlBits */
+    Q1T_ERR_INVALID_NR_MEASUREMENT_BITS = 5,  /* Error::InvalidNrMeasurementBits */
+    Q1T_ERR_NOT_ENOUGH_SPACE = 6,             /* Error::NotEnoughSpace */
+    Q1T_ERR_NOT_EXECUTED = 7,                 /* Error::NotExecuted */
+    Q1T_ERR_INVALID_WEIGHTS = 8,              /* WeightedIndex::new failure (vectorstate.rs:123-125) */
+    Q1T_ERR_UNKNOWN_GATE = 9,                 /* ParseError::UnknownGate */
+    Q1T_ERR_INVALID_NR_ARGUMENTS = 10,        /* ParseError::InvalidNrArguments */
+    Q1T_ERR_HIP = 11,
+    Q1T_ERR_OUT_OF_MEMORY = 12,
+    Q1T_ERR_INVALID_ARGUMENT = 13,
+    Q1T_ERR_NOT_SUPPORTED = 14
+};
+
+enum { Q1T_PREC_C128 = 0, Q1T_PREC_C64 = 1 };
+
+typedef struct q1t_gvs q1t_gvs;
+
+/* Random source: raw 64-bit draws (a Rust `R: Rng` behind a trampoline, or
+ * q1t_xoshiro_next).  Uniforms are (x >> 12) * 2^-52, as rand 0.7 builds a
+ * Uniform<f64>; a Binomial(count, p) draw is the number of runs with u < p. */
+typedef struct {
+    void* ctx;
+    uint64_t (*next_u64)(void* ctx);
+} q1t_rng;
+
+typedef struct {
+    double gate_seconds;   /* wall time of the last gate flush */
+    double bytes;          /* algorithmic HBM bytes moved by it */
+    uint64_t launches;     /* kernels launched */
+    uint64_t sweeps;       /* HBM sweeps over the state */
+    uint64_t gates;        /* gates applied */
+} q1t_stats;
+
+const char* q1t_last_error(void);
+
+/* VectorState::new (vectorstate.rs:41-53): |0...0>, one column, nr_shots runs. */
+int q1t_gvs_new(size_t nr_bits, size_t nr_shots, int precision, q1t_gvs** out);
+void q1t_gvs_free(q1t_gvs* s);
+/* VectorState::from_qubit_coefs (vectorstate.rs:62-83): 2*nr_bits complex coefficients. */
+int q1t_gvs_from_qubit_coefs(q1t_gvs* s, const double* coefs, size_t ncoefs);
+/* Build-added: upload a 2^n amplitude vector (no reference counterpart). */
+int q1t_gvs_set_amplitudes(q1t_gvs* s, const double* amps, size_t len);
+/* Build-added: re, im i.i.d. N(0,1) (SplitMix64 counter hash + Box-Muller), normalised. */
+int q1t_gvs_set_random_normal(q1t_gvs* s, uint64_t seed);
+
+size_t q1t_gvs_nr_bits(const q1t_gvs* s);
+size_t q1t_gvs_nr_cols(const q1t_gvs* s);
+int q1t_gvs_counts(const q1t_gvs* s, size_t* out, size_t len);
+
+/* QuState::apply_gate (qustate.rs:8, vectorstate.rs:166-178). */
+int q1t_gvs_apply_gate(q1t_gvs* s, const double* mat, size_t nr_affected, const size_t* bits, size_t nbits,
+                       const char* desc);
+/* QuState::apply_unary_gate_all (qustate.rs:13, vectorstate.rs:180-189). */
+int q1t_gvs_apply_unary_gate_all(q1t_gvs* s, const double* mat2, const char* desc);
+/* QuState::apply_conditional_gate (qustate.rs:17, vectorstate.rs:193-227). */
+int q1t_gvs_apply_conditional_gate(q1t_gvs* s, const uint8_t* control, size_t ncontrol, const double* mat,
+                                   size_t nr_affected, const size_t* bits, size_t nbits, const char* desc);
+
+/* Two-step measure for hosts that draw their own binomials
+ * (vectorstate.rs:249-275 then 277-327): w0 per column, then the collapse. */
+int q1t_gvs_marginal_w0(q1t_gvs* s, size_t qbit, double* w0s, size_t len);
+int q1t_gvs_measure_with_n0(q1t_gvs* s, size_t qbit, size_t cbit, const size_t* n0s, const double* w0s,
+                            size_t ncols, uint64_t* res, size_t res_len, int collapse);
+/* QuState::measure_into (qustate.rs:35, vectorstate.rs:237-329). */
+int q1t_gvs_measure_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len, const q1t_rng* rng);
+/* QuState::peek_into (qustate.rs:62, vectorstate.rs:346-393). */
+int q1t_gvs_peek_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len, const q1t_rng* rng);
+/* QuState::measure_all_into (qustate.rs:50, vectorstate.rs:340-344). */
+int q1t_gvs_measure_all_into(q1t_gvs* s, const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,
+                             const q1t_rng* rng);
+/* QuState::peek_all_into (qustate.rs:74, vectorstate.rs:395-400). */
+int q1t_gvs_peek_all_into(q1t_gvs* s, const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,
+                          const q1t_rng* rng);
+/* measure_all_into_helper (vectorstate.rs:106-161) with one injected uniform
+ * in [0,1) per run: the chosen weight is u * total, mapped to an index by
+ * the canonical F=16 tree walk (FP64).  collapse = 0 is peek_all. */
+int q1t_gvs_measure_all_uniforms(q1t_gvs* s, const size_t* cbits, size_t ncbits, const double* u01,
+                                 uint64_t* res, size_t res_len, int collapse);
+/* QuState::reset (qustate.rs:82, vectorstate.rs:402-408). */
+int q1t_gvs_reset(q1t_gvs* s, size_t bit, const q1t_rng* rng);
+/* QuState::reset_all (qustate.rs:88, vectorstate.rs:410-415). */
+int q1t_gvs_reset_all(q1t_gvs* s);
+
+/* Build-added read-out: column `col` as 2^n interleaved complex doubles. */
+int q1t_gvs_read_amplitudes(q1t_gvs* s, size_t col, double* out, size_t len);
+/* Build-added: apply queued gates now. */
+int q1t_gvs_flush(q1t_gvs* s);
+/* Build-added: fusion_k = 0 keeps per-gate FP64/FP32 arithmetic (plain);
+ * 2..6 composes gate runs into dense k-qubit blocks; tiling = 1 applies
+ * runs that fit an LDS tile in one HBM sweep. */
+int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling);
+int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out);
+
+/* Build-added measurement: with profiling on, every kernel launch of a flush
+ * is bracketed by HIP events on the state's stream; per-kernel totals. */
+typedef struct {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    double bytes;      /* algorithmic HBM bytes of those launches */
+} q1t_kernel_profile;
+int q1t_gvs_set_profiling(q1t_gvs* s, int on);
+int q1t_gvs_kernel_profile(const q1t_gvs* s, q1t_kernel_profile* out, size_t cap, size_t* count);
+
+/* Build-added accuracy report: <a|b>, max_i |a_i - b_i| and both squared
+ * norms over one column of each state, FP64 on the device (either state may
+ * be complex64).  Infidelity = 1 - |<a|b>|^2 / (|a|^2 |b|^2). */
+int q1t_gvs_compare(q1t_gvs* a, size_t col_a, q1t_gvs* b, size_t col_b, double* out5);
+
+/* Gate::matrix() of a named gate (names of ffi.rs:335-366 and
+ * composite.rs:289-445, case-insensitive): writes 2^k x 2^k interleaved
+ * complex doubles (out must hold 2 * 4^k doubles, cap counts doubles) and k. */
+int q1t_gate_matrix(const char* name, const double* params, size_t nparams, double* out, size_t cap,
+                    size_t* nbits);
+
+/* Built-in xoshiro256** stream, seeded through SplitMix64. */
+typedef struct q1t_xoshiro q1t_xoshiro;
+q1t_xoshiro* q1t_xoshiro_new(uint64_t seed);
+void q1t_xoshiro_free(q1t_xoshiro* x);
+uint64_t q1t_xoshiro_next(void* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

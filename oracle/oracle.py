@@ -92,11 +92,13 @@ def lib():
             "orc_reset": (i32, [vp, sz, vp]),
             "orc_reset_all": (None, [vp]),
             "orc_tree_sample": (sz, [dp, sz, ctypes.c_double, dp]),
+            "orc_tree_sample_many": (None, [dp, sz, dp, sz, dp, dp]),
             "orc_weighted_index_seq": (sz, [dp, sz, ctypes.c_double]),
             "orc_reverse_bits": (ctypes.c_uint64, [ctypes.c_uint64, sz]),
             "orc_shuffle_bits": (ctypes.c_uint64, [ctypes.c_uint64, dp, sz]),
             "orc_collect_conditional_ranges": (sz, [dp, sz, dp, dp, dp, dp]),
             "orc_random_normal_state": (None, [dp, sz, ctypes.c_uint64]),
+            "orc_canonical_weights": (None, [dp, sz, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -260,6 +262,27 @@ def tree_sample(w, u):
     tot = ctypes.c_double(0.0)
     idx = lib().orc_tree_sample(_ptr(w), len(w), float(u), ctypes.byref(tot))
     return int(idx), tot.value
+
+
+def tree_sample_many(w, us):
+    """Canonical tree walk for many uniforms over one weight vector."""
+    w = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    us = np.ascontiguousarray(np.asarray(us, dtype=np.float64))
+    out = np.zeros(len(us), dtype=np.uintp)
+    tot = ctypes.c_double(0.0)
+    lib().orc_tree_sample_many(_ptr(w), len(w), _ptr(us), len(us), _ptr(out), ctypes.byref(tot))
+    return out.astype(np.uint64), tot.value
+
+
+def canonical_weights(amps):
+    """fma(re, re, im*im) per amplitude, as orc_norm_sqr computes it."""
+    a = np.asarray(amps, dtype=np.complex128)
+    out = np.zeros(len(a), dtype=np.float64)
+    L = lib()
+    # vectorised equivalent is not bit-exact (numpy has no fma); use the C loop
+    c = np.ascontiguousarray(a)
+    L.orc_canonical_weights(_ptr(c), len(c), _ptr(out))
+    return out
 
 
 def weighted_index_seq(w, u):

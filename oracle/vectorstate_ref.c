@@ -671,6 +671,14 @@ size_t orc_weighted_index_seq(const double* w, size_t len, double u)
  */
 size_t orc_tree_sample(const double* w, size_t len, double u, double* total_out)
 {
+    size_t idx = 0;
+    orc_tree_sample_many(w, len, &u, 1, &idx, total_out);
+    return idx;
+}
+
+void orc_tree_sample_many(const double* w, size_t len, const double* us, size_t nu, size_t* out,
+                          double* total_out)
+{
     const size_t F = ORC_TREE_FANOUT;
     size_t nlev = 1;
     size_t sizes[64];
@@ -698,28 +706,51 @@ size_t orc_tree_sample(const double* w, size_t len, double u, double* total_out)
     double total = lev[top][0];
     for (size_t k = 1; k < sizes[top]; ++k) total += lev[top][k];
     if (total_out) *total_out = total;
-    double t = u * total;
-    double offset = 0.0;
-    size_t gstart = 0, glen = sizes[top], idx = 0;
-    for (size_t l = top + 1; l-- > 0;) {
-        const double* v = lev[l] + gstart;
-        size_t last = 0;
-        for (size_t j = 0; j < glen; ++j)
-            if (v[j] > 0.0) last = j;
-        size_t chosen = last;
-        for (size_t j = 0; j < last; ++j) {
-            double nx = offset + v[j];
-            if (nx <= t) offset = nx;
-            else { chosen = j; break; }
+    for (size_t r = 0; r < nu; ++r) {
+        double t = us[r] * total;
+        double offset = 0.0;
+        size_t gstart = 0, glen = sizes[top], idx = 0;
+        for (size_t l = top + 1; l-- > 0;) {
+            const double* v = lev[l] + gstart;
+            size_t last = 0;
+            for (size_t j = 0; j < glen; ++j)
+                if (v[j] > 0.0) last = j;
+            size_t chosen = last;
+            for (size_t j = 0; j < last; ++j) {
+                double nx = offset + v[j];
+                if (nx <= t) offset = nx;
+                else { chosen = j; break; }
+            }
+            idx = gstart + chosen;
+            if (l == 0) break;
+            gstart = idx * F;
+            size_t rem = sizes[l - 1] - gstart;
+            glen = rem < F ? rem : F;
         }
-        idx = gstart + chosen;
-        if (l == 0) break;
-        gstart = idx * F;
-        size_t rem = sizes[l - 1] - gstart;
-        glen = rem < F ? rem : F;
+        out[r] = idx;
     }
     for (size_t l = 1; l < nlev; ++l) free(owned[l]);
-    return idx;
+}
+
+/* rand 0.7 WeightedIndex on many uniforms with one cumulative table. */
+static void weighted_index_seq_many(const double* w, size_t len, const double* us, size_t nu, size_t* out)
+{
+    double total = w[0];
+    double* cw = (double*)malloc((len > 1 ? len - 1 : 1) * sizeof(double));
+    for (size_t i = 1; i < len; ++i) {
+        cw[i - 1] = total;
+        total += w[i];
+    }
+    for (size_t r = 0; r < nu; ++r) {
+        double t = us[r] * total;
+        size_t lo = 0, hi = len - 1;
+        while (lo < hi) {
+            size_t mid = lo + (hi - lo) / 2;
+            if (cw[mid] <= t) lo = mid + 1; else hi = mid;
+        }
+        out[r] = lo;
+    }
+    free(cw);
 }
 
 /* reverse_bits / shuffle_bits (support.rs:50-75) */
@@ -775,9 +806,10 @@ int orc_measure_all_uniforms(orc_state* s, const size_t* cbits, size_t ncbits,
             return ORC_ERR_WEIGHTS;
         }
         size_t count = s->counts[c];
-        for (size_t r = 0; r < count; ++r)
-            samples[run + r] = mode == 0 ? orc_tree_sample(w, N, u01[run + r], NULL)
-                                         : orc_weighted_index_seq(w, N, u01[run + r]);
+        if (mode == 0)
+            orc_tree_sample_many(w, N, u01 + run, count, samples + run, NULL);
+        else
+            weighted_index_seq_many(w, N, u01 + run, count, samples + run);
         qsort(samples + run, count, sizeof(size_t), cmp_size);
         for (size_t r = 0; r < count;) {
             size_t e = r;
@@ -881,4 +913,10 @@ void orc_random_normal_state(orc_cplx* out, size_t nr_bits, uint64_t seed)
     }
     double f = 1.0 / sqrt(ss);
     for (size_t i = 0; i < N; ++i) { out[i].re *= f; out[i].im *= f; }
+}
+
+/* Canonical sampling weights of an amplitude vector (orc_norm_sqr each). */
+void orc_canonical_weights(const orc_cplx* a, size_t n, double* out)
+{
+    for (size_t i = 0; i < n; ++i) out[i] = orc_norm_sqr(a[i]);
 }

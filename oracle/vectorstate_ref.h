@@ -99,9 +99,12 @@ void orc_reset_all(orc_state* s);
 
 /* Sampling primitives, exported for unit tests. */
 #define ORC_TREE_FANOUT 16
-size_t orc_tree_sample(const double* w, size_t len, double t_over_total_u, double* total_out);
+size_t orc_tree_sample(const double* w, size_t len, double u, double* total_out);
+void   orc_tree_sample_many(const double* w, size_t len, const double* us, size_t nu, size_t* out,
+                            double* total_out);
 size_t orc_weighted_index_seq(const double* w, size_t len, double u);
 double orc_norm_sqr(orc_cplx a);
+void   orc_canonical_weights(const orc_cplx* a, size_t n, double* out);
 
 /* Support bit helpers (support.rs:50-75). */
 uint64_t orc_reverse_bits(uint64_t idx, size_t nr_bits);

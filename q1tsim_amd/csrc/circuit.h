@@ -1,0 +1,87 @@
+// Circuit: op list + interpreter, mirroring q1tsim's Circuit (src/circuit.rs)
+// with the GPU statevector as its quantum state.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gvs.h"
+
+namespace q1t {
+
+// Parameter::{Direct, FFIRef} (gates/parameter.rs:24-57): read at apply time.
+struct Param {
+    double value = 0.0;
+    const double* ptr = nullptr;
+    double get() const { return ptr ? *ptr : value; }
+};
+
+enum class OpType { Gate, CondGate, Measure, MeasureAll, Peek, PeekAll, Reset, ResetAll, Barrier };
+
+struct CircuitOp {
+    OpType type;
+    const GateSpec* gate = nullptr;
+    std::vector<Param> params;
+    std::vector<size_t> bits;       // qubits of a gate / barrier
+    std::vector<size_t> control;    // classical control bits (ConditionalGate)
+    uint64_t target = 0;
+    size_t qbit = 0, cbit = 0;
+    std::vector<size_t> cbits;      // MeasureAll / PeekAll
+    char basis = 'Z';
+};
+
+class Circuit {
+public:
+    Circuit(size_t nr_qbits, size_t nr_cbits) : nr_qbits_(nr_qbits), nr_cbits_(nr_cbits) {}
+
+    size_t nr_qbits() const { return nr_qbits_; }
+    size_t nr_cbits() const { return nr_cbits_; }
+    const std::vector<uint64_t>* cstate() const { return has_cstate_ ? &c_state_ : nullptr; }
+
+    void add_gate(const std::string& name, const size_t* qbits, size_t n, const Param* params, size_t np);
+    void add_conditional_gate(const size_t* control, size_t nc, uint64_t target, const std::string& name,
+                              const size_t* qbits, size_t n, const Param* params, size_t np);
+    void measure_basis(size_t qbit, size_t cbit, char basis, bool collapse);
+    void measure_all_basis(const size_t* cbits, size_t n, char basis, bool collapse);
+    void reset(size_t qbit);
+    void reset_all();
+    void barrier(const size_t* qbits, size_t n);
+
+    void execute(size_t nr_shots);
+    void reexecute();
+
+    std::vector<std::pair<std::string, size_t>> histogram_string() const;
+    std::vector<size_t> histogram_vec() const;
+
+    // build-added
+    void set_backend(Precision prec, int fusion_k, bool tiling);
+    void seed(uint64_t s) { rng_.reseed(s); }
+    void set_initial_amplitudes(std::vector<cdouble> amps);
+    void set_initial_random(uint64_t seed);
+    GpuVectorState* state() { return state_.get(); }
+    const GpuVectorState* state() const { return state_.get(); }
+    const RunStats& last_stats() const { return stats_; }
+
+private:
+    size_t nr_qbits_, nr_cbits_;
+    std::vector<CircuitOp> ops_;
+    std::unique_ptr<GpuVectorState> state_;
+    std::vector<uint64_t> c_state_;
+    bool has_cstate_ = false;
+    Xoshiro256 rng_{0x1f67a51423cd2615ULL};
+    Precision prec_ = PREC_C128;
+    int fusion_k_ = 0;
+    bool tiling_ = true;
+    int init_mode_ = 0;            // 0: |0..0>, 1: amplitudes, 2: random normal
+    std::vector<cdouble> init_amps_;
+    uint64_t init_seed_ = 0;
+    RunStats stats_;
+
+    void do_execute();
+    void apply_named(GpuVectorState& st, const char* name, size_t q);
+    void apply_named_all(GpuVectorState& st, const char* name);
+};
+
+}  // namespace q1t

@@ -1,0 +1,153 @@
+// GpuVectorState: the MI355X statevector behind q1tsim's QuState interface
+// (src/qustate.rs:5-89), replacing VectorState (src/vectorstate.rs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+
+#include "common.h"
+#include "gates.h"
+
+namespace q1t {
+
+enum Precision { PREC_C128 = 0, PREC_C64 = 1 };
+
+// Per-state counters of the work the last flush did (for bandwidth reports).
+struct RunStats {
+    double gate_seconds = 0.0;    // wall time of the last flush (host clock)
+    double bytes = 0.0;           // algorithmic HBM bytes of the launched sweeps
+    uint64_t launches = 0;        // kernels launched
+    uint64_t sweeps = 0;          // full-state passes
+    uint64_t gates = 0;           // gates applied
+};
+
+// Kernel time measured with HIP events on the state's stream.
+struct KernelProfile {
+    std::string name;
+    uint64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;   // algorithmic HBM bytes of those launches
+};
+
+// <a|b>, max |a_i - b_i| and the two norms, accumulated in FP64.
+struct StateComparison {
+    double overlap_re = 0, overlap_im = 0, max_abs = 0, norm_a = 0, norm_b = 0;
+};
+
+struct PendingGate {
+    GateOp op;
+    std::vector<cdouble> matrix;  // full FP64 matrix (fused path composes these)
+    std::vector<int> qbits;       // index-bit positions of the listed qubits
+};
+
+class GpuVectorState {
+public:
+    GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec);
+    ~GpuVectorState();
+    GpuVectorState(const GpuVectorState&) = delete;
+    GpuVectorState& operator=(const GpuVectorState&) = delete;
+
+    size_t nr_bits() const { return nr_bits_; }
+    size_t nr_shots() const { return nr_shots_; }
+    size_t nr_cols() const { return counts_.size(); }
+    const std::vector<size_t>& counts() const { return counts_; }
+    Precision precision() const { return prec_; }
+    bool is_basis() const { return basis_; }
+
+    // initial states
+    void set_zero_state();                                   // VectorState::new
+    void set_product_state(const cdouble* coefs);            // from_qubit_coefs
+    void set_amplitudes(const cdouble* amps);                // build-added upload
+    void set_random_normal(uint64_t seed);                   // N(0,1) re/im, normalised
+
+    // QuState
+    void apply_gate(const cdouble* mat, size_t nr_affected, const size_t* qbits, size_t nbits,
+                    const std::string& desc);
+    void apply_unary_gate_all(const cdouble* mat2, const std::string& desc);
+    void apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
+                                size_t nr_affected, const size_t* qbits, size_t nbits, const std::string& desc);
+    std::vector<double> marginal_w0(size_t qbit);
+    void measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
+                         uint64_t* res, size_t res_len, bool collapse);
+    void measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng, bool collapse);
+    void measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
+                              size_t res_len, bool collapse);
+    void measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len, RngSource& rng,
+                          bool collapse);
+    void reset(size_t bit, RngSource& rng);
+    void reset_all();
+
+    // build-added
+    void read_amplitudes(size_t col, cdouble* out);
+    void flush();
+    void synchronize();
+    const RunStats& last_stats() const { return stats_; }
+    void set_fusion(int max_block_bits) { fusion_k_ = max_block_bits; }
+    void set_tiling(bool on) { tiling_ = on; }
+    hipStream_t stream() const { return stream_; }
+    void* device_amplitudes() { return amp_; }
+    size_t elem_bytes() const { return prec_ == PREC_C128 ? 16 : 8; }
+    void* col_ptr(size_t c) const;
+    // Device scratch buffers kept between calls, one per use; growing one
+    // synchronises the stream first.
+    enum ScratchSlot { SLOT_MATRIX = 0, SLOT_PROGRAM = 1, SLOT_FUSED = 2, SLOT_COUNT = 3 };
+    void* scratch(int slot, size_t bytes);
+    void* scratch_matrix(size_t bytes) { return scratch(SLOT_MATRIX, bytes); }
+    void apply_op(const GateOp& op, size_t col_begin, size_t col_end);
+
+    // kernel profiling (off by default): every launch of a flush is bracketed
+    // by HIP events; the times are gathered after the flush synchronises.
+    void set_profiling(bool on);
+    int prof_begin();
+    void prof_end(int slot, const char* name, double bytes);
+    std::vector<KernelProfile> profile() const;
+    static StateComparison compare(GpuVectorState& a, size_t col_a, GpuVectorState& b, size_t col_b);
+
+private:
+    size_t nr_bits_, nr_shots_;
+    Precision prec_;
+    std::vector<size_t> counts_;
+    bool basis_ = false;                  // columns are basis vectors |basis_idx_[c]>
+    std::vector<uint64_t> basis_idx_;
+    void* amp_ = nullptr;                 // column-major: column c at amp_ + c * 2^n elements
+    size_t amp_cols_ = 0;                 // allocated columns
+    hipStream_t stream_ = nullptr;
+    std::vector<PendingGate> queue_;
+    RunStats stats_;
+    int fusion_k_ = 0;                    // 0: plain per-gate arithmetic
+    bool tiling_ = false;
+
+    // scratch
+    double* d_partial_ = nullptr;
+    size_t partial_cap_ = 0;
+    double* d_tree_ = nullptr;
+    size_t tree_cap_ = 0;
+    void* d_small_ = nullptr;
+    size_t small_cap_ = 0;
+    void* d_slot_[SLOT_COUNT] = {nullptr, nullptr, nullptr};
+    size_t slot_cap_[SLOT_COUNT] = {0, 0, 0};
+
+    bool profiling_ = false;
+    struct PendingEvent {
+        int pair;
+        std::string name;
+        double bytes;
+    };
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+    size_t ev_used_ = 0;
+    std::vector<PendingEvent> ev_pending_;
+    std::vector<KernelProfile> prof_;
+    void prof_collect();
+
+    uint64_t dim() const { return uint64_t(1) << nr_bits_; }
+    void check_qbit(size_t q) const;
+    void alloc_cols(size_t ncols);
+    void* small_buf(size_t bytes);
+    double* partial_buf(size_t n);
+    void materialize();
+    void apply_queue_plain(size_t col_begin, size_t col_end);
+    void sample_column(size_t col, const double* u01, size_t nruns, uint64_t* out_idx);
+};
+
+}  // namespace q1t

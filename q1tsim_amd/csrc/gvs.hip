@@ -1,0 +1,1306 @@
+// GpuVectorState: state storage, per-gate sweep kernels and measurement
+// kernels for MI355X (gfx950).
+//
+// Layout in HBM: one contiguous 2^n-amplitude vector per branch column
+// (column-major), interleaved complex (re, im) in FP64 (c128) or FP32 (c64).
+// The reference stores the same matrix row-major (cmatrix.rs:20-24); a column
+// per contiguous vector makes every gate sweep a unit-stride pass over one
+// column and lets columns be added, dropped and copied without a transpose.
+#include "gvs.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.h"
+#include "sweep.h"
+
+namespace q1t {
+
+#define HIP_OK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw Error(e_ == hipErrorOutOfMemory ? ERR_OUT_OF_MEMORY : ERR_HIP,                    \
+                        std::string("HIP error in " #expr ": ") + hipGetErrorString(e_));          \
+    } while (0)
+
+// ------------------------------------------------------------------ kernels
+
+// One-qubit dense update on every (i0, i0|tbit) pair whose control bits are 1.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_dense1(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                   PosList pos, uint64_t cmask, uint64_t tbit, V m0, V m1, V m2, V m3)
+{
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i0 = deposit(j, pos) | cmask, i1 = i0 | tbit;
+        const V x = a[i0], y = a[i1];
+        a[i0] = cadd(cmul(m0, x), cmul(m1, y));
+        a[i1] = cadd(cmul(m2, x), cmul(m3, y));
+    }
+}
+
+template <typename V> struct Mat4 {
+    V m[16];
+};
+
+// Two-qubit dense update; t0 is the local MSB.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_dense2(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                   PosList pos, uint64_t cmask, uint64_t b0, uint64_t b1, Mat4<V> m)
+{
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i = deposit(j, pos) | cmask;
+        const uint64_t ix[4] = {i, i | b1, i | b0, i | b0 | b1};
+        V v[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) v[l] = a[ix[l]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            V acc = cmul(m.m[r * 4], v[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) acc = cmac(acc, m.m[r * 4 + c], v[c]);
+            a[ix[r]] = acc;
+        }
+    }
+}
+
+// K-qubit dense update (K = 3..5), matrix staged in LDS.
+template <typename V, int K>
+__global__ void __launch_bounds__(kBlock) k_denseK(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                   PosList pos, uint64_t cmask, Targets tg, const V* __restrict__ mat)
+{
+    constexpr int D = 1 << K;
+    __shared__ V sm[D * D];
+    for (int t = threadIdx.x; t < D * D; t += blockDim.x) sm[t] = mat[t];
+    __syncthreads();
+    uint64_t off[D];
+#pragma unroll
+    for (int l = 0; l < D; ++l) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if ((l >> (K - 1 - q)) & 1) o |= uint64_t(1) << tg.t[q];
+        off[l] = o;
+    }
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i = deposit(j, pos) | cmask;
+        V v[D];
+#pragma unroll
+        for (int l = 0; l < D; ++l) v[l] = a[i | off[l]];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            V acc = cmul(sm[r * D], v[0]);
+#pragma unroll
+            for (int c = 1; c < D; ++c) acc = cmac(acc, sm[r * D + c], v[c]);
+            a[i | off[r]] = acc;
+        }
+    }
+}
+
+// Dense update for 6..10 target qubits: one workgroup per amplitude group,
+// the group staged in LDS, matrix rows read from global memory.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_dense_big(V* __restrict__ amp, uint64_t col_stride, uint64_t ngroups,
+                                                      PosList pos, uint64_t cmask, Targets tg,
+                                                      const V* __restrict__ mat)
+{
+    extern __shared__ unsigned char smem_raw[];
+    V* sg = reinterpret_cast<V*>(smem_raw);
+    const int K = tg.cnt, D = 1 << K;
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const uint64_t base = deposit(g, pos) | cmask;
+        for (int l = threadIdx.x; l < D; l += blockDim.x) {
+            uint64_t o = 0;
+            for (int q = 0; q < K; ++q)
+                if ((l >> (K - 1 - q)) & 1) o |= uint64_t(1) << tg.t[q];
+            sg[l] = a[base | o];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < D; r += blockDim.x) {
+            V acc = mkc<V>(0, 0);
+            for (int c = 0; c < D; ++c) acc = cmac(acc, mat[uint64_t(r) * D + c], sg[c]);
+            uint64_t o = 0;
+            for (int q = 0; q < K; ++q)
+                if ((r >> (K - 1 - q)) & 1) o |= uint64_t(1) << tg.t[q];
+            a[base | o] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename V> struct Diag8 {
+    V d[8];
+};
+
+// Diagonal gate on up to 3 targets: a[i] *= d[local(i)].
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_diag(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                 PosList pos, uint64_t cmask, Targets tg, Diag8<V> dv)
+{
+    __shared__ V sd[8];
+    if (threadIdx.x < 8) sd[threadIdx.x] = dv.d[threadIdx.x];
+    __syncthreads();
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i = deposit(j, pos) | cmask;
+        int l = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (q < tg.cnt) l = (l << 1) | int((i >> tg.t[q]) & 1);
+        a[i] = cmul(sd[l], a[i]);
+    }
+}
+
+// Phase on the subspace where every bit of `mask` is 1 (Z, S, T, U1, CZ, CU1 ...).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_phase_ones(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                       PosList pos, uint64_t mask, V ph)
+{
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i = deposit(j, pos) | mask;
+        a[i] = cmul(ph, a[i]);
+    }
+}
+
+// Swap: exchange |..0a..1b..> and |..1a..0b..> (controls set).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_swap2(V* __restrict__ amp, uint64_t col_stride, uint64_t nwork,
+                                                  PosList pos, uint64_t cmask, uint64_t ba, uint64_t bb)
+{
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i = deposit(j, pos) | cmask;
+        const V x = a[i | bb], y = a[i | ba];
+        a[i | bb] = y;
+        a[i | ba] = x;
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double red[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < int(blockDim.x) / 64; ++k) s += red[k];
+    return s;
+}
+
+// Partial sums of |a|^2 over the subspace given by pos (FP64).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_norm_partial(const V* __restrict__ amp, uint64_t col_stride,
+                                                         uint64_t nwork, PosList pos, double* __restrict__ partial)
+{
+    const V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    double acc = 0.0;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const V v = a[deposit(j, pos)];
+        acc += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[uint64_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// Level sums of the canonical sampling tree: out[g] = left fold of
+// in[16g .. 16g+15].  A workgroup folds 4096 inputs; loads are coalesced
+// into LDS in a transposed, padded layout so each thread reads its 16
+// values bank-conflict free.
+template <typename T> __device__ __forceinline__ double leaf_value(const T* in, uint64_t k);
+template <> __device__ __forceinline__ double leaf_value<double2>(const double2* in, uint64_t k) { return weight(in[k]); }
+template <> __device__ __forceinline__ double leaf_value<float2>(const float2* in, uint64_t k) { return weight(in[k]); }
+template <> __device__ __forceinline__ double leaf_value<double>(const double* in, uint64_t k) { return in[k]; }
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_tree_level(const T* __restrict__ in, uint64_t size, double* __restrict__ out)
+{
+    __shared__ double w[kTreeFanout * (kBlock + 1)];
+    const uint64_t nout = (size + kTreeFanout - 1) / kTreeFanout;
+    const uint64_t chunk = uint64_t(kBlock) * kTreeFanout;
+    for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < size; base += uint64_t(gridDim.x) * chunk) {
+#pragma unroll
+        for (int r = 0; r < kTreeFanout; ++r) {
+            const int e = r * kBlock + threadIdx.x;
+            const uint64_t k = base + e;
+            const double v = k < size ? leaf_value<T>(in, k) : 0.0;
+            w[(e & (kTreeFanout - 1)) * (kBlock + 1) + (e >> 4)] = v;
+        }
+        __syncthreads();
+        const uint64_t g = base / kTreeFanout + threadIdx.x;
+        if (g < nout) {
+            double acc = w[threadIdx.x];
+            for (int jj = 1; jj < kTreeFanout; ++jj)
+                if (g * kTreeFanout + jj < size) acc += w[jj * (kBlock + 1) + threadIdx.x];
+            out[g] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+struct TreeDesc {
+    int nlev;
+    uint64_t size[16];
+    const double* lev[16];   // lev[0] unused: level 0 is read from the amplitudes
+};
+
+template <typename V>
+__device__ __forceinline__ double tree_value(const V* a, const TreeDesc& td, int l, uint64_t k)
+{
+    return l == 0 ? weight(a[k]) : td.lev[l][k];
+}
+
+// Inverse-CDF sampling of one run per thread by the canonical tree walk:
+// t = u * T; within each group keep offset += S_j while offset + S_j <= t,
+// descend into the first child where that fails, or the last positive one.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_tree_sample(const V* __restrict__ a, TreeDesc td,
+                                                        const double* __restrict__ u01, uint64_t nruns,
+                                                        uint64_t* __restrict__ out)
+{
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    const int top = td.nlev - 1;
+    double total = tree_value(a, td, top, 0);
+    for (uint64_t k = 1; k < td.size[top]; ++k) total += tree_value(a, td, top, k);
+    const double t = u01[r] * total;
+    double offset = 0.0;
+    uint64_t gstart = 0, glen = td.size[top], idx = 0;
+    for (int l = top; l >= 0; --l) {
+        double v[kTreeFanout];
+        int last = 0;
+#pragma unroll
+        for (int j = 0; j < kTreeFanout; ++j) {
+            v[j] = (uint64_t(j) < glen) ? tree_value(a, td, l, gstart + j) : 0.0;
+            if (v[j] > 0.0) last = j;
+        }
+        int chosen = last;
+        for (int j = 0; j < last; ++j) {
+            const double nx = offset + v[j];
+            if (nx <= t) {
+                offset = nx;
+            } else {
+                chosen = j;
+                break;
+            }
+        }
+        idx = gstart + uint64_t(chosen);
+        if (l == 0) break;
+        gstart = idx * kTreeFanout;
+        const uint64_t rem = td.size[l - 1] - gstart;
+        glen = rem < uint64_t(kTreeFanout) ? rem : uint64_t(kTreeFanout);
+    }
+    out[r] = idx;
+}
+
+// Fused collapse + renormalise of one column onto qubit bit p:
+// dst0 keeps the bit-0 half scaled by s0, dst1 the bit-1 half scaled by s1
+// (either may be null; src may alias either destination).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_collapse(const V* src, V* dst0, V* dst1, uint64_t nwork, int p,
+                                                     typename RealOf<V>::type s0, typename RealOf<V>::type s1)
+{
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t bit = uint64_t(1) << p;
+    const V zero = mkc<V>(0, 0);
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1)), i1 = i0 | bit;
+        const V x0 = src[i0], x1 = src[i1];
+        if (dst0) {
+            dst0[i0] = mkc<V>(x0.x * s0, x0.y * s0);
+            dst0[i1] = zero;
+        }
+        if (dst1) {
+            dst1[i0] = zero;
+            dst1[i1] = mkc<V>(x1.x * s1, x1.y * s1);
+        }
+    }
+}
+
+// Product state: amplitude = b_{n-1}[x_{n-1}] * (... * (b_0[x_0] * 1)), the
+// association order of successive kron_mat calls (vectorstate.rs:68-74).
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_product(V* __restrict__ amp, uint64_t N, int n, const double2* __restrict__ b)
+{
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += step) {
+        double re = 1.0, im = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const double2 c = b[2 * q + ((i >> (n - 1 - q)) & 1)];
+            const double nr = c.x * re - c.y * im, ni = c.x * im + c.y * re;
+            re = nr;
+            im = ni;
+        }
+        amp[i] = mkc<V>(typename RealOf<V>::type(re), typename RealOf<V>::type(im));
+    }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// re, im ~ N(0,1): SplitMix64 counter hash of (seed, i) and Box-Muller.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_random_normal(V* __restrict__ amp, uint64_t N, uint64_t seed)
+{
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += step) {
+        const uint64_t ha = mix64(seed + 0x9e3779b97f4a7c15ULL * (2 * i + 1));
+        const uint64_t hb = mix64(seed + 0x9e3779b97f4a7c15ULL * (2 * i + 2));
+        const double u1 = double((ha >> 11) + 1) * (1.0 / 9007199254740992.0);
+        const double u2 = double(hb >> 11) * (1.0 / 9007199254740992.0);
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 2.0 * M_PI * u2;
+        amp[i] = mkc<V>(typename RealOf<V>::type(r * cos(th)), typename RealOf<V>::type(r * sin(th)));
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_scale(V* __restrict__ amp, uint64_t N, double f)
+{
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += step) {
+        const V v = amp[i];
+        amp[i] = mkc<V>(typename RealOf<V>::type(v.x * f), typename RealOf<V>::type(v.y * f));
+    }
+}
+
+// ------------------------------------------------------------- host helpers
+
+static unsigned grid_for(uint64_t nwork)
+{
+    uint64_t g = (nwork + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return unsigned(g);
+}
+
+static PosList make_pos(std::vector<int> v)
+{
+    std::sort(v.begin(), v.end());
+    PosList p{};
+    p.cnt = int(v.size());
+    for (size_t k = 0; k < v.size(); ++k) p.s[k] = v[k];
+    return p;
+}
+
+static std::vector<int> mask_bits(uint64_t m)
+{
+    std::vector<int> r;
+    for (int b = 0; b < 64; ++b)
+        if ((m >> b) & 1) r.push_back(b);
+    return r;
+}
+
+template <typename V> static V to_v(const cdouble& c)
+{
+    return mkc<V>(typename RealOf<V>::type(c.real()), typename RealOf<V>::type(c.imag()));
+}
+
+// ------------------------------------------------------------ the state
+
+GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec)
+    : nr_bits_(nr_bits), nr_shots_(nr_shots), prec_(prec)
+{
+    if (nr_bits > 40) throw Error(ERR_INVALID_ARGUMENT, "Too many qubits for a dense state");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw Error(ERR_HIP, "No HIP device available for the GPU statevector");
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    set_zero_state();
+}
+
+GpuVectorState::~GpuVectorState()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (amp_) (void)hipFree(amp_);
+    if (d_partial_) (void)hipFree(d_partial_);
+    if (d_tree_) (void)hipFree(d_tree_);
+    if (d_small_) (void)hipFree(d_small_);
+    for (int s = 0; s < SLOT_COUNT; ++s)
+        if (d_slot_[s]) (void)hipFree(d_slot_[s]);
+    for (auto& ev : ev_pool_) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void GpuVectorState::check_qbit(size_t q) const
+{
+    if (q >= nr_bits_) throw err_invalid_qbit(q);
+}
+
+void* GpuVectorState::col_ptr(size_t c) const
+{
+    return static_cast<char*>(amp_) + c * dim() * elem_bytes();
+}
+
+void GpuVectorState::alloc_cols(size_t ncols)
+{
+    if (ncols <= amp_cols_ && amp_) return;
+    HIP_OK(hipStreamSynchronize(stream_));
+    if (amp_) {
+        HIP_OK(hipFree(amp_));
+        amp_ = nullptr;
+        amp_cols_ = 0;
+    }
+    HIP_OK(hipMalloc(&amp_, ncols * dim() * elem_bytes()));
+    amp_cols_ = ncols;
+}
+
+void* GpuVectorState::small_buf(size_t bytes)
+{
+    if (bytes > small_cap_) {
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (d_small_) HIP_OK(hipFree(d_small_));
+        size_t cap = std::max<size_t>(bytes, 1 << 20);
+        HIP_OK(hipMalloc(&d_small_, cap));
+        small_cap_ = cap;
+    }
+    return d_small_;
+}
+
+double* GpuVectorState::partial_buf(size_t n)
+{
+    if (n > partial_cap_) {
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (d_partial_) HIP_OK(hipFree(d_partial_));
+        HIP_OK(hipMalloc(&d_partial_, n * sizeof(double)));
+        partial_cap_ = n;
+    }
+    return d_partial_;
+}
+
+void GpuVectorState::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
+
+void GpuVectorState::set_zero_state()
+{
+    queue_.clear();
+    basis_ = false;
+    counts_.assign(1, nr_shots_);
+    alloc_cols(1);
+    HIP_OK(hipMemsetAsync(amp_, 0, dim() * elem_bytes(), stream_));
+    if (prec_ == PREC_C128) {
+        const double one[2] = {1.0, 0.0};
+        HIP_OK(hipMemcpyAsync(amp_, one, sizeof(one), hipMemcpyHostToDevice, stream_));
+    } else {
+        const float one[2] = {1.0f, 0.0f};
+        HIP_OK(hipMemcpyAsync(amp_, one, sizeof(one), hipMemcpyHostToDevice, stream_));
+    }
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void GpuVectorState::set_product_state(const cdouble* coefs)
+{
+    queue_.clear();
+    basis_ = false;
+    counts_.assign(1, nr_shots_);
+    alloc_cols(1);
+    // normalise each qubit as vectorstate.rs:71-72 does
+    std::vector<double> b(4 * nr_bits_);
+    for (size_t q = 0; q < nr_bits_; ++q) {
+        cdouble c0 = coefs[2 * q], c1 = coefs[2 * q + 1];
+        double norm = std::sqrt((c0.real() * c0.real() + c0.imag() * c0.imag()) +
+                                (c1.real() * c1.real() + c1.imag() * c1.imag()));
+        b[4 * q + 0] = c0.real() / norm;
+        b[4 * q + 1] = c0.imag() / norm;
+        b[4 * q + 2] = c1.real() / norm;
+        b[4 * q + 3] = c1.imag() / norm;
+    }
+    void* d = small_buf(b.size() * sizeof(double));
+    HIP_OK(hipMemcpyAsync(d, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (prec_ == PREC_C128)
+        hipLaunchKernelGGL(k_product<double2>, dim3(grid_for(dim())), dim3(kBlock), 0, stream_,
+                           static_cast<double2*>(amp_), dim(), int(nr_bits_), static_cast<const double2*>(d));
+    else
+        hipLaunchKernelGGL(k_product<float2>, dim3(grid_for(dim())), dim3(kBlock), 0, stream_,
+                           static_cast<float2*>(amp_), dim(), int(nr_bits_), static_cast<const double2*>(d));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void GpuVectorState::set_amplitudes(const cdouble* amps)
+{
+    queue_.clear();
+    basis_ = false;
+    counts_.assign(1, nr_shots_);
+    alloc_cols(1);
+    if (prec_ == PREC_C128) {
+        HIP_OK(hipMemcpy(amp_, amps, dim() * sizeof(cdouble), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(2 * dim());
+        for (uint64_t i = 0; i < dim(); ++i) {
+            f[2 * i] = float(amps[i].real());
+            f[2 * i + 1] = float(amps[i].imag());
+        }
+        HIP_OK(hipMemcpy(amp_, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+}
+
+void GpuVectorState::set_random_normal(uint64_t seed)
+{
+    queue_.clear();
+    basis_ = false;
+    counts_.assign(1, nr_shots_);
+    alloc_cols(1);
+    const unsigned g = grid_for(dim());
+    PosList none{};
+    double* part = partial_buf(512);
+    if (prec_ == PREC_C128) {
+        hipLaunchKernelGGL(k_random_normal<double2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<double2*>(amp_), dim(), seed);
+        hipLaunchKernelGGL(k_norm_partial<double2>, dim3(512), dim3(kBlock), 0, stream_, static_cast<const double2*>(amp_),
+                           dim(), dim(), none, part);
+    } else {
+        hipLaunchKernelGGL(k_random_normal<float2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<float2*>(amp_), dim(), seed);
+        hipLaunchKernelGGL(k_norm_partial<float2>, dim3(512), dim3(kBlock), 0, stream_, static_cast<const float2*>(amp_),
+                           dim(), dim(), none, part);
+    }
+    HIP_OK(hipGetLastError());
+    std::vector<double> h(512);
+    HIP_OK(hipMemcpyAsync(h.data(), part, 512 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    double ss = 0.0;
+    for (double v : h) ss += v;
+    const double f = 1.0 / std::sqrt(ss);
+    if (prec_ == PREC_C128)
+        hipLaunchKernelGGL(k_scale<double2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<double2*>(amp_), dim(), f);
+    else
+        hipLaunchKernelGGL(k_scale<float2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<float2*>(amp_), dim(), f);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void GpuVectorState::materialize()
+{
+    if (!basis_) return;
+    const size_t nc = counts_.size();
+    const size_t bytes_needed = nc * dim() * elem_bytes();
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    size_t have = amp_ ? amp_cols_ * dim() * elem_bytes() : 0;
+    if (bytes_needed > free_b + have)
+        throw Error(ERR_OUT_OF_MEMORY, "Not enough device memory to expand " + std::to_string(nc) +
+                                           " measured basis columns into dense columns");
+    alloc_cols(nc);
+    HIP_OK(hipMemsetAsync(amp_, 0, nc * dim() * elem_bytes(), stream_));
+    for (size_t c = 0; c < nc; ++c) {
+        char* p = static_cast<char*>(col_ptr(c)) + basis_idx_[c] * elem_bytes();
+        if (prec_ == PREC_C128) {
+            static const double one[2] = {1.0, 0.0};
+            HIP_OK(hipMemcpyAsync(p, one, sizeof(one), hipMemcpyHostToDevice, stream_));
+        } else {
+            static const float one[2] = {1.0f, 0.0f};
+            HIP_OK(hipMemcpyAsync(p, one, sizeof(one), hipMemcpyHostToDevice, stream_));
+        }
+    }
+    HIP_OK(hipStreamSynchronize(stream_));
+    basis_ = false;
+    basis_idx_.clear();
+}
+
+// --------------------------------------------------------------- gates
+
+void GpuVectorState::apply_gate(const cdouble* mat, size_t nr_affected, const size_t* qbits, size_t nbits,
+                                const std::string& desc)
+{
+    if (nr_affected != nbits) throw err_invalid_nr_bits(nbits, nr_affected, desc);
+    for (size_t k = 0; k < nbits; ++k) check_qbit(qbits[k]);
+    if (nbits > 10) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 10 qubits are not supported");
+    PendingGate pg;
+    pg.op = analyze_gate(mat, nbits, qbits, nr_bits_);
+    const size_t d = size_t(1) << nbits;
+    pg.matrix.assign(mat, mat + d * d);
+    for (size_t k = 0; k < nbits; ++k) pg.qbits.push_back(int(nr_bits_ - 1 - qbits[k]));
+    queue_.push_back(std::move(pg));
+}
+
+void GpuVectorState::apply_unary_gate_all(const cdouble* mat2, const std::string& desc)
+{
+    for (size_t q = 0; q < nr_bits_; ++q) apply_gate(mat2, 1, &q, 1, desc);
+}
+
+void GpuVectorState::flush()
+{
+    if (queue_.empty()) return;
+    materialize();
+    auto t0 = std::chrono::steady_clock::now();
+    stats_ = RunStats{};
+    if (fusion_k_ > 0 || tiling_)
+        run_planned(*this, queue_, 0, counts_.size(), fusion_k_, tiling_, stats_);
+    else
+        apply_queue_plain(0, counts_.size());
+    HIP_OK(hipStreamSynchronize(stream_));
+    stats_.gate_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    queue_.clear();
+    prof_collect();
+}
+
+void GpuVectorState::apply_queue_plain(size_t cb, size_t ce)
+{
+    for (const PendingGate& pg : queue_) apply_op(pg.op, cb, ce);
+}
+
+template <typename V>
+static void launch_op(const GateOp& op, V* amp, uint64_t col_stride, size_t ncols, int n, hipStream_t st,
+                      GpuVectorState& gs, RunStats& stats)
+{
+    if (op.kind == OpKind::Identity) {
+        stats.gates += 1;
+        return;
+    }
+    const std::vector<int> ctrl = mask_bits(op.ctrl_mask);
+    const double S = double(uint64_t(1) << n) * sizeof(V);
+    const double frac = 1.0 / double(uint64_t(1) << ctrl.size());   // touched fraction
+    for (size_t c0 = 0; c0 < ncols; c0 += 65535) {
+        const unsigned ny = unsigned(std::min<size_t>(65535, ncols - c0));
+        V* a = amp + c0 * col_stride;
+        const char* kname = "";
+        double bytes = 2.0 * S * frac * ny;
+        int ps = -1;
+        switch (op.kind) {
+        case OpKind::PhaseOnes: {
+            const uint64_t nwork = uint64_t(1) << (n - int(ctrl.size()));
+            kname = "k_phase_ones";
+            ps = gs.prof_begin();
+            hipLaunchKernelGGL(k_phase_ones<V>, dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride, nwork,
+                               make_pos(ctrl), op.ctrl_mask, to_v<V>(op.m[0]));
+            break;
+        }
+        case OpKind::Diag: {
+            const uint64_t nwork = uint64_t(1) << (n - int(ctrl.size()));
+            Targets tg{};
+            tg.cnt = int(op.targets.size());
+            for (int q = 0; q < tg.cnt; ++q) tg.t[q] = op.targets[q];
+            Diag8<V> dv{};
+            for (size_t l = 0; l < op.m.size(); ++l) dv.d[l] = to_v<V>(op.m[l]);
+            kname = "k_diag";
+            ps = gs.prof_begin();
+            hipLaunchKernelGGL(k_diag<V>, dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride, nwork,
+                               make_pos(ctrl), op.ctrl_mask, tg, dv);
+            break;
+        }
+        case OpKind::Swap2: {
+            std::vector<int> pos = ctrl;
+            pos.push_back(op.targets[0]);
+            pos.push_back(op.targets[1]);
+            const uint64_t nwork = uint64_t(1) << (n - int(pos.size()));
+            kname = "k_swap2";
+            bytes *= 0.5;
+            ps = gs.prof_begin();
+            hipLaunchKernelGGL(k_swap2<V>, dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride, nwork,
+                               make_pos(pos), op.ctrl_mask, uint64_t(1) << op.targets[0], uint64_t(1) << op.targets[1]);
+            break;
+        }
+        case OpKind::Dense: {
+            const int kt = int(op.targets.size());
+            std::vector<int> pos = ctrl;
+            for (int t : op.targets) pos.push_back(t);
+            const uint64_t nwork = uint64_t(1) << (n - int(pos.size()));
+            const PosList pl = make_pos(pos);
+            if (kt == 1) {
+                kname = "k_dense1";
+                ps = gs.prof_begin();
+                hipLaunchKernelGGL(k_dense1<V>, dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride, nwork,
+                                   pl, op.ctrl_mask, uint64_t(1) << op.targets[0], to_v<V>(op.m[0]),
+                                   to_v<V>(op.m[1]), to_v<V>(op.m[2]), to_v<V>(op.m[3]));
+            } else if (kt == 2) {
+                Mat4<V> m4{};
+                for (int k = 0; k < 16; ++k) m4.m[k] = to_v<V>(op.m[k]);
+                kname = "k_dense2";
+                ps = gs.prof_begin();
+                hipLaunchKernelGGL(k_dense2<V>, dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride, nwork,
+                                   pl, op.ctrl_mask, uint64_t(1) << op.targets[0], uint64_t(1) << op.targets[1], m4);
+            } else {
+                Targets tg{};
+                tg.cnt = kt;
+                for (int q = 0; q < kt; ++q) tg.t[q] = op.targets[q];
+                const size_t d = size_t(1) << kt;
+                std::vector<V> hm(d * d);
+                for (size_t k = 0; k < d * d; ++k) hm[k] = to_v<V>(op.m[k]);
+                V* dm = static_cast<V*>(gs.scratch_matrix(hm.size() * sizeof(V)));
+                if (hipMemcpyAsync(dm, hm.data(), hm.size() * sizeof(V), hipMemcpyHostToDevice, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)
+                    throw Error(ERR_HIP, "matrix upload failed");
+                kname = kt <= 5 ? "k_denseK" : "k_dense_big";
+                ps = gs.prof_begin();
+                if (kt == 3)
+                    hipLaunchKernelGGL((k_denseK<V, 3>), dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride,
+                                       nwork, pl, op.ctrl_mask, tg, dm);
+                else if (kt == 4)
+                    hipLaunchKernelGGL((k_denseK<V, 4>), dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride,
+                                       nwork, pl, op.ctrl_mask, tg, dm);
+                else if (kt == 5)
+                    hipLaunchKernelGGL((k_denseK<V, 5>), dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride,
+                                       nwork, pl, op.ctrl_mask, tg, dm);
+                else
+                    hipLaunchKernelGGL(k_dense_big<V>, dim3(grid_for(nwork * kBlock), ny), dim3(kBlock), d * sizeof(V),
+                                       st, a, col_stride, nwork, pl, op.ctrl_mask, tg, dm);
+            }
+            break;
+        }
+        default:
+            break;
+        }
+        if (hipGetLastError() != hipSuccess) throw Error(ERR_HIP, "gate kernel launch failed");
+        gs.prof_end(ps, kname, bytes);
+        stats.bytes += bytes;
+        stats.launches += 1;
+    }
+    stats.gates += 1;
+    stats.sweeps += 1;
+}
+
+void* GpuVectorState::scratch(int slot, size_t bytes)
+{
+    if (bytes > slot_cap_[slot]) {
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (d_slot_[slot]) HIP_OK(hipFree(d_slot_[slot]));
+        d_slot_[slot] = nullptr;
+        slot_cap_[slot] = 0;
+        const size_t cap = std::max<size_t>(bytes, 1 << 16);
+        HIP_OK(hipMalloc(&d_slot_[slot], cap));
+        slot_cap_[slot] = cap;
+    }
+    return d_slot_[slot];
+}
+
+void GpuVectorState::apply_op(const GateOp& op, size_t cb, size_t ce)
+{
+    if (ce <= cb) return;
+    if (prec_ == PREC_C128)
+        launch_op<double2>(op, static_cast<double2*>(col_ptr(cb)), dim(), ce - cb, int(nr_bits_), stream_, *this, stats_);
+    else
+        launch_op<float2>(op, static_cast<float2*>(col_ptr(cb)), dim(), ce - cb, int(nr_bits_), stream_, *this, stats_);
+}
+
+void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
+                                            size_t nr_affected, const size_t* qbits, size_t nbits,
+                                            const std::string& desc)
+{
+    if (ncontrol != nr_shots_) throw err_invalid_nr_control_bits(ncontrol, nr_shots_, desc);
+    if (nr_affected != nbits) throw err_invalid_nr_bits(nbits, nr_affected, desc);
+    for (size_t k = 0; k < nbits; ++k) check_qbit(qbits[k]);
+    flush();
+    // collect_conditional_ranges (qustate.rs:100-127)
+    struct Range {
+        size_t col, len;
+        bool apply;
+    };
+    std::vector<Range> ranges;
+    size_t off = 0;
+    for (size_t icol = 0; icol < counts_.size(); ++icol) {
+        const size_t count = counts_[icol];
+        if (count == 0) continue;
+        size_t begin = off;
+        bool prev = control[off] != 0;
+        for (size_t ib = off + 1; ib < off + count; ++ib) {
+            const bool cb = control[ib] != 0;
+            if (cb != prev) {
+                ranges.push_back({icol, ib - begin, prev});
+                begin = ib;
+                prev = !prev;
+            }
+        }
+        if (begin < off + count) ranges.push_back({icol, off + count - begin, prev});
+        off += count;
+    }
+    bool any = false;
+    for (const Range& r : ranges) any = any || r.apply;
+    bool same_layout = ranges.size() == counts_.size();
+    for (size_t r = 0; same_layout && r < ranges.size(); ++r) same_layout = ranges[r].col == r;
+    if (basis_ && same_layout && !any) return;
+    materialize();
+    GateOp op = analyze_gate(mat, nbits, qbits, nr_bits_);
+    if (!same_layout) {
+        // new column set: one column per range, copied from its source column
+        const size_t nr = ranges.size();
+        void* old = amp_;
+        const size_t old_cols = amp_cols_;
+        void* fresh = nullptr;
+        HIP_OK(hipMalloc(&fresh, nr * dim() * elem_bytes()));
+        for (size_t r = 0; r < nr; ++r)
+            HIP_OK(hipMemcpyAsync(static_cast<char*>(fresh) + r * dim() * elem_bytes(), col_ptr(ranges[r].col),
+                                  dim() * elem_bytes(), hipMemcpyDeviceToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        HIP_OK(hipFree(old));
+        (void)old_cols;
+        amp_ = fresh;
+        amp_cols_ = nr;
+    }
+    counts_.clear();
+    for (const Range& r : ranges) counts_.push_back(r.len);
+    for (size_t r = 0; r < ranges.size();) {
+        if (!ranges[r].apply) {
+            ++r;
+            continue;
+        }
+        size_t e = r;
+        while (e < ranges.size() && ranges[e].apply) ++e;
+        apply_op(op, r, e);
+        r = e;
+    }
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------- measurement
+
+std::vector<double> GpuVectorState::marginal_w0(size_t qbit)
+{
+    check_qbit(qbit);
+    flush();
+    const size_t nc = counts_.size();
+    std::vector<double> w0(nc, 0.0);
+    const int p = int(nr_bits_ - 1 - qbit);
+    if (basis_) {
+        for (size_t c = 0; c < nc; ++c) w0[c] = ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+        return w0;
+    }
+    const unsigned nb = 256;
+    double* part = partial_buf(size_t(nb) * nc);
+    const PosList pl = make_pos({p});
+    const uint64_t nwork = dim() / 2;
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
+        if (prec_ == PREC_C128)
+            hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nb, ny), dim3(kBlock), 0, stream_,
+                               static_cast<const double2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * nb);
+        else
+            hipLaunchKernelGGL(k_norm_partial<float2>, dim3(nb, ny), dim3(kBlock), 0, stream_,
+                               static_cast<const float2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * nb);
+        HIP_OK(hipGetLastError());
+    }
+    std::vector<double> h(size_t(nb) * nc);
+    HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (size_t c = 0; c < nc; ++c)
+        for (unsigned b = 0; b < nb; ++b) w0[c] += h[c * nb + b];
+    return w0;
+}
+
+void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
+                                     uint64_t* res, size_t res_len, bool collapse)
+{
+    check_qbit(qbit);
+    if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
+    const size_t nc = counts_.size();
+    const uint64_t one_mask = uint64_t(1) << cbit, zero_mask = ~one_mask;
+    size_t rs = 0, new_nr = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        const size_t n0 = std::min(n0s[c], counts_[c]);
+        for (size_t r = rs; r < rs + n0; ++r) res[r] &= zero_mask;
+        for (size_t r = rs + n0; r < rs + counts_[c]; ++r) res[r] |= one_mask;
+        rs += counts_[c];
+        new_nr += (n0 == 0 || n0 == counts_[c]) ? 1 : 2;
+    }
+    if (!collapse) return;
+    const int p = int(nr_bits_ - 1 - qbit);
+    if (basis_ && new_nr == nc) return;   // a basis column collapses onto itself
+    materialize();
+    const uint64_t nwork = dim() / 2;
+    auto launch = [&](const void* src, void* d0, void* d1, double s0, double s1) {
+        if (prec_ == PREC_C128)
+            hipLaunchKernelGGL(k_collapse<double2>, dim3(grid_for(nwork)), dim3(kBlock), 0, stream_,
+                               static_cast<const double2*>(src), static_cast<double2*>(d0), static_cast<double2*>(d1),
+                               nwork, p, s0, s1);
+        else
+            hipLaunchKernelGGL(k_collapse<float2>, dim3(grid_for(nwork)), dim3(kBlock), 0, stream_,
+                               static_cast<const float2*>(src), static_cast<float2*>(d0), static_cast<float2*>(d1),
+                               nwork, p, float(s0), float(s1));
+        HIP_OK(hipGetLastError());
+    };
+    std::vector<size_t> ncounts;
+    if (new_nr == nc) {
+        for (size_t c = 0; c < nc; ++c) {
+            const size_t n0 = std::min(n0s[c], counts_[c]);
+            void* col = col_ptr(c);
+            if (n0 == counts_[c])
+                launch(col, col, nullptr, 1.0 / std::sqrt(w0s[c]), 0.0);
+            else
+                launch(col, nullptr, col, 0.0, 1.0 / std::sqrt(1.0 - w0s[c]));
+            ncounts.push_back(counts_[c]);
+        }
+    } else {
+        void* fresh = nullptr;
+        HIP_OK(hipMalloc(&fresh, new_nr * dim() * elem_bytes()));
+        auto fcol = [&](size_t k) { return static_cast<char*>(fresh) + k * dim() * elem_bytes(); };
+        size_t ni = 0;
+        for (size_t c = 0; c < nc; ++c) {
+            const size_t n0 = std::min(n0s[c], counts_[c]), count = counts_[c];
+            const double w0 = w0s[c];
+            if (n0 == count) {
+                launch(col_ptr(c), fcol(ni), nullptr, 1.0 / std::sqrt(w0), 0.0);
+                ncounts.push_back(count);
+                ni += 1;
+            } else if (n0 == 0) {
+                launch(col_ptr(c), nullptr, fcol(ni), 0.0, 1.0 / std::sqrt(1.0 - w0));
+                ncounts.push_back(count);
+                ni += 1;
+            } else {
+                launch(col_ptr(c), fcol(ni), fcol(ni + 1), 1.0 / std::sqrt(w0), 1.0 / std::sqrt(1.0 - w0));
+                ncounts.push_back(n0);
+                ncounts.push_back(count - n0);
+                ni += 2;
+            }
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        HIP_OK(hipFree(amp_));
+        amp_ = fresh;
+        amp_cols_ = new_nr;
+    }
+    counts_ = ncounts;
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng,
+                                  bool collapse)
+{
+    check_qbit(qbit);
+    if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
+    std::vector<double> w0 = marginal_w0(qbit);
+    // n0 ~ Binomial(count, min(w0, 1)) drawn as a count of Bernoulli trials,
+    // one uniform per run, columns in order (vectorstate.rs:264-275).
+    std::vector<size_t> n0s(counts_.size());
+    for (size_t c = 0; c < counts_.size(); ++c) {
+        const double p = w0[c] < 1.0 ? w0[c] : 1.0;
+        size_t n0 = 0;
+        for (size_t r = 0; r < counts_[c]; ++r)
+            if (rng.uniform() < p) ++n0;
+        n0s[c] = n0;
+    }
+    measure_with_n0(qbit, cbit, n0s.data(), w0, res, res_len, collapse);
+}
+
+void GpuVectorState::sample_column(size_t col, const double* u01, size_t nruns, uint64_t* out_idx)
+{
+    // levels of the canonical tree; level sizes shrink by 16 until <= 16
+    TreeDesc td{};
+    td.nlev = 1;
+    td.size[0] = dim();
+    uint64_t total_entries = 0;
+    while (td.size[td.nlev - 1] > uint64_t(kTreeFanout)) {
+        td.size[td.nlev] = (td.size[td.nlev - 1] + kTreeFanout - 1) / kTreeFanout;
+        total_entries += td.size[td.nlev];
+        td.nlev += 1;
+    }
+    if (total_entries > tree_cap_) {
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (d_tree_) HIP_OK(hipFree(d_tree_));
+        HIP_OK(hipMalloc(&d_tree_, total_entries * sizeof(double)));
+        tree_cap_ = total_entries;
+    }
+    uint64_t o = 0;
+    for (int l = 1; l < td.nlev; ++l) {
+        td.lev[l] = d_tree_ + o;
+        o += td.size[l];
+    }
+    const void* a = col_ptr(col);
+    for (int l = 1; l < td.nlev; ++l) {
+        const uint64_t chunks = (td.size[l - 1] + uint64_t(kBlock) * kTreeFanout - 1) / (uint64_t(kBlock) * kTreeFanout);
+        const unsigned g = unsigned(std::min<uint64_t>(chunks, 65536));
+        double* outp = const_cast<double*>(td.lev[l]);
+        if (l == 1) {
+            if (prec_ == PREC_C128)
+                hipLaunchKernelGGL(k_tree_level<double2>, dim3(g), dim3(kBlock), 0, stream_,
+                                   static_cast<const double2*>(a), td.size[0], outp);
+            else
+                hipLaunchKernelGGL(k_tree_level<float2>, dim3(g), dim3(kBlock), 0, stream_,
+                                   static_cast<const float2*>(a), td.size[0], outp);
+        } else {
+            hipLaunchKernelGGL(k_tree_level<double>, dim3(g), dim3(kBlock), 0, stream_, td.lev[l - 1], td.size[l - 1],
+                               outp);
+        }
+        HIP_OK(hipGetLastError());
+    }
+    // all-zero or NaN weights make WeightedIndex::new fail (vectorstate.rs:123-125)
+    {
+        const int top = td.nlev - 1;
+        const size_t nt = size_t(td.size[top]);
+        std::vector<double> tv(nt);
+        if (top == 0) {
+            std::vector<cdouble> av(nt);
+            if (prec_ == PREC_C128) {
+                HIP_OK(hipMemcpyAsync(av.data(), a, nt * sizeof(cdouble), hipMemcpyDeviceToHost, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+            } else {
+                std::vector<float> f(2 * nt);
+                HIP_OK(hipMemcpyAsync(f.data(), a, f.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+                for (size_t k = 0; k < nt; ++k) av[k] = cdouble(f[2 * k], f[2 * k + 1]);
+            }
+            for (size_t k = 0; k < nt; ++k) tv[k] = std::norm(av[k]);
+        } else {
+            HIP_OK(hipMemcpyAsync(tv.data(), td.lev[top], nt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
+        double total = 0.0;
+        for (double v : tv) total += v;
+        if (!(total > 0.0) || !std::isfinite(total))
+            throw Error(ERR_INVALID_WEIGHTS, "Cannot sample from a state with zero or invalid norm");
+    }
+    // uniforms in, indices out
+    const size_t ub = nruns * sizeof(double);
+    char* buf = static_cast<char*>(small_buf(2 * ub + 64));
+    double* du = reinterpret_cast<double*>(buf);
+    uint64_t* dout = reinterpret_cast<uint64_t*>(buf + ub);
+    HIP_OK(hipMemcpyAsync(du, u01, ub, hipMemcpyHostToDevice, stream_));
+    const unsigned g = unsigned((nruns + kBlock - 1) / kBlock);
+    if (prec_ == PREC_C128)
+        hipLaunchKernelGGL(k_tree_sample<double2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<const double2*>(a),
+                           td, du, uint64_t(nruns), dout);
+    else
+        hipLaunchKernelGGL(k_tree_sample<float2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<const float2*>(a),
+                           td, du, uint64_t(nruns), dout);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out_idx, dout, nruns * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
+                                          size_t res_len, bool collapse)
+{
+    if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
+    if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
+    flush();
+    const size_t nc = counts_.size();
+    std::vector<uint64_t> uidx, ucnt;
+    std::vector<uint64_t> samples;
+    size_t run = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        const size_t count = counts_[c];
+        if (count == 0) continue;
+        if (basis_) {
+            uidx.push_back(basis_idx_[c]);
+            ucnt.push_back(count);
+            run += count;
+            continue;
+        }
+        samples.assign(count, 0);
+        sample_column(c, u01 + run, count, samples.data());
+        std::sort(samples.begin(), samples.end());
+        for (size_t r = 0; r < count;) {
+            size_t e = r;
+            while (e < count && samples[e] == samples[r]) ++e;
+            uidx.push_back(samples[r]);
+            ucnt.push_back(e - r);
+            r = e;
+        }
+        run += count;
+    }
+    uint64_t mask = 0;
+    for (size_t k = 0; k < ncbits; ++k) mask |= uint64_t(1) << cbits[k];
+    mask = ~mask;
+    size_t off = 0;
+    for (size_t k = 0; k < uidx.size(); ++k) {
+        const uint64_t perm = shuffle_bits(reverse_bits(uidx[k], nr_bits_), cbits, ncbits);
+        for (size_t r = off; r < off + ucnt[k]; ++r) res[r] = (res[r] & mask) | perm;
+        off += ucnt[k];
+    }
+    if (collapse) {
+        basis_ = true;
+        basis_idx_ = uidx;
+        counts_.assign(ucnt.begin(), ucnt.end());
+    }
+}
+
+void GpuVectorState::measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,
+                                      RngSource& rng, bool collapse)
+{
+    if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
+    if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
+    std::vector<double> u(nr_shots_);
+    for (size_t r = 0; r < nr_shots_; ++r) u[r] = rng.uniform();
+    measure_all_uniforms(cbits, ncbits, u.data(), res, res_len, collapse);
+}
+
+void GpuVectorState::reset(size_t bit, RngSource& rng)
+{
+    check_qbit(bit);
+    std::vector<uint64_t> m(nr_shots_, 0);
+    measure_into(bit, 0, m.data(), m.size(), rng, true);
+    std::vector<uint8_t> control(nr_shots_);
+    for (size_t r = 0; r < nr_shots_; ++r) control[r] = m[r] != 0;
+    const cdouble x[4] = {0.0, 1.0, 1.0, 0.0};
+    apply_conditional_gate(control.data(), control.size(), x, 1, &bit, 1, "X");
+}
+
+void GpuVectorState::reset_all() { set_zero_state(); }
+
+void GpuVectorState::read_amplitudes(size_t col, cdouble* out)
+{
+    flush();
+    if (col >= counts_.size()) throw Error(ERR_INVALID_ARGUMENT, "Column index out of range");
+    if (basis_) {
+        std::fill(out, out + dim(), cdouble(0, 0));
+        out[basis_idx_[col]] = cdouble(1, 0);
+        return;
+    }
+    if (prec_ == PREC_C128) {
+        HIP_OK(hipMemcpyAsync(out, col_ptr(col), dim() * sizeof(cdouble), hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<float> f(2 * dim());
+        HIP_OK(hipMemcpyAsync(f.data(), col_ptr(col), f.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (uint64_t i = 0; i < dim(); ++i) out[i] = cdouble(f[2 * i], f[2 * i + 1]);
+    }
+}
+
+
+// ------------------------------------------------------------ profiling
+
+void GpuVectorState::set_profiling(bool on)
+{
+    profiling_ = on;
+    prof_.clear();
+    ev_pending_.clear();
+    ev_used_ = 0;
+}
+
+int GpuVectorState::prof_begin()
+{
+    if (!profiling_) return -1;
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t a, b;
+        HIP_OK(hipEventCreate(&a));
+        HIP_OK(hipEventCreate(&b));
+        ev_pool_.emplace_back(a, b);
+    }
+    const int slot = int(ev_used_++);
+    HIP_OK(hipEventRecord(ev_pool_[slot].first, stream_));
+    return slot;
+}
+
+void GpuVectorState::prof_end(int slot, const char* name, double bytes)
+{
+    if (slot < 0) return;
+    HIP_OK(hipEventRecord(ev_pool_[slot].second, stream_));
+    ev_pending_.push_back(PendingEvent{slot, name, bytes});
+}
+
+void GpuVectorState::prof_collect()
+{
+    if (ev_pending_.empty()) return;
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (const PendingEvent& pe : ev_pending_) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, ev_pool_[pe.pair].first, ev_pool_[pe.pair].second));
+        KernelProfile* kp = nullptr;
+        for (KernelProfile& k : prof_)
+            if (k.name == pe.name) kp = &k;
+        if (!kp) {
+            prof_.push_back(KernelProfile{pe.name, 0, 0.0, 0.0});
+            kp = &prof_.back();
+        }
+        kp->launches += 1;
+        kp->ms += ms;
+        kp->bytes += pe.bytes;
+    }
+    ev_pending_.clear();
+    ev_used_ = 0;
+}
+
+std::vector<KernelProfile> GpuVectorState::profile() const { return prof_; }
+
+// ------------------------------------------------------------ comparison
+
+template <typename VA, typename VB>
+__global__ void __launch_bounds__(kBlock) k_compare(const VA* __restrict__ a, const VB* __restrict__ b, uint64_t N,
+                                                    double* __restrict__ out)
+{
+    double ore = 0, oim = 0, mx = 0, na = 0, nb = 0;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += step) {
+        const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+        ore += ar * br + ai * bi;   // conj(a) * b
+        oim += ar * bi - ai * br;
+        na += ar * ar + ai * ai;
+        nb += br * br + bi * bi;
+        const double dr = ar - br, di = ai - bi;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+    const double s0 = block_sum(ore);
+    __syncthreads();
+    const double s1 = block_sum(oim);
+    __syncthreads();
+    const double s2 = block_sum(na);
+    __syncthreads();
+    const double s3 = block_sum(nb);
+    __shared__ double red[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        for (int k = 0; k < int(blockDim.x) / 64; ++k) m = fmax(m, red[k]);
+        double* o = out + 5 * blockIdx.x;
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = m;
+        o[3] = s2;
+        o[4] = s3;
+    }
+}
+
+StateComparison GpuVectorState::compare(GpuVectorState& a, size_t col_a, GpuVectorState& b, size_t col_b)
+{
+    if (a.nr_bits_ != b.nr_bits_) throw Error(ERR_INVALID_ARGUMENT, "States differ in qubit count");
+    a.flush();
+    b.flush();
+    a.materialize();
+    b.materialize();
+    if (col_a >= a.counts_.size() || col_b >= b.counts_.size())
+        throw Error(ERR_INVALID_ARGUMENT, "Column index out of range");
+    HIP_OK(hipStreamSynchronize(b.stream_));
+    const unsigned nb = 1024;
+    double* part = a.partial_buf(5 * nb);
+    const uint64_t N = a.dim();
+    const void* pa = a.col_ptr(col_a);
+    const void* pb = b.col_ptr(col_b);
+    const bool a64 = a.prec_ == PREC_C128, b64 = b.prec_ == PREC_C128;
+    if (a64 && b64)
+        hipLaunchKernelGGL((k_compare<double2, double2>), dim3(nb), dim3(kBlock), 0, a.stream_,
+                           static_cast<const double2*>(pa), static_cast<const double2*>(pb), N, part);
+    else if (a64)
+        hipLaunchKernelGGL((k_compare<double2, float2>), dim3(nb), dim3(kBlock), 0, a.stream_,
+                           static_cast<const double2*>(pa), static_cast<const float2*>(pb), N, part);
+    else if (b64)
+        hipLaunchKernelGGL((k_compare<float2, double2>), dim3(nb), dim3(kBlock), 0, a.stream_,
+                           static_cast<const float2*>(pa), static_cast<const double2*>(pb), N, part);
+    else
+        hipLaunchKernelGGL((k_compare<float2, float2>), dim3(nb), dim3(kBlock), 0, a.stream_,
+                           static_cast<const float2*>(pa), static_cast<const float2*>(pb), N, part);
+    HIP_OK(hipGetLastError());
+    std::vector<double> h(5 * nb);
+    HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, a.stream_));
+    HIP_OK(hipStreamSynchronize(a.stream_));
+    StateComparison r;
+    for (unsigned k = 0; k < nb; ++k) {
+        r.overlap_re += h[5 * k];
+        r.overlap_im += h[5 * k + 1];
+        r.max_abs = std::max(r.max_abs, h[5 * k + 2]);
+        r.norm_a += h[5 * k + 3];
+        r.norm_b += h[5 * k + 4];
+    }
+    return r;
+}
+
+}  // namespace q1t

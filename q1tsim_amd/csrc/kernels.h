@@ -1,0 +1,72 @@
+// Device helpers and the per-gate / measurement kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace q1t {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 waves)
+constexpr int kTreeFanout = 16;   // F of the canonical sampling tree
+
+// Sorted (ascending) index-bit positions at which zero bits are deposited.
+struct PosList {
+    int cnt;
+    int s[16];
+};
+
+struct Targets {
+    int cnt;
+    int t[12];   // t[0] = most significant local bit
+};
+
+template <typename V> struct RealOf;
+template <> struct RealOf<double2> { using type = double; };
+template <> struct RealOf<float2> { using type = float; };
+
+template <typename V> __host__ __device__ __forceinline__ V mkc(typename RealOf<V>::type re, typename RealOf<V>::type im)
+{
+    V v;
+    v.x = re;
+    v.y = im;
+    return v;
+}
+template <typename V> __device__ __forceinline__ V cmul(V a, V b)
+{
+    return mkc<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return mkc<V>(a.x + b.x, a.y + b.y); }
+// acc + a*b
+template <typename V> __device__ __forceinline__ V cmac(V acc, V a, V b)
+{
+    return mkc<V>(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+}
+
+// Deposit zero bits of j at the positions in p (ascending): maps a work index
+// to a state index whose bits at those positions are 0.
+// The loop is unrolled over the fixed capacity so the position array is
+// indexed with constants (kernel-argument arrays indexed dynamically would
+// be copied to scratch memory).
+__device__ __forceinline__ uint64_t deposit(uint64_t j, const PosList& p)
+{
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < p.cnt) {
+            const int s = p.s[k];
+            const uint64_t lo = j & ((uint64_t(1) << s) - 1);
+            j = ((j >> s) << (s + 1)) | lo;
+        }
+    }
+    return j;
+}
+
+// Canonical sampling weight |a|^2 = fma(re, re, im*im) in FP64 (the same
+// operation order as the CPU oracle's orc_norm_sqr).
+template <typename V> __device__ __forceinline__ double weight(V v)
+{
+    const double re = double(v.x), im = double(v.y);
+    return __fma_rn(re, re, __dmul_rn(im, im));
+}
+
+}  // namespace q1t

@@ -1,0 +1,61 @@
+// Gate-stream planner and multi-gate sweeps (LDS/register tiles, fused blocks).
+#pragma once
+
+#include <vector>
+
+#include "gvs.h"
+
+namespace q1t {
+
+// One in-tile operation of a sweep program (read by every thread of a
+// workgroup; uniform).
+enum SwOpType : int32_t { SW_DENSE1 = 0, SW_PHASE = 1, SW_DIAG1 = 2, SW_BLOCK = 3 };
+
+struct SwOp {
+    int32_t type;
+    int32_t reg;       // SW_DENSE1: index of the target among the phase's register bits
+    int32_t dlocal;    // SW_DIAG1: local bit of the target, or -1
+    int32_t dglobal;   // SW_DIAG1: global bit of the target when it is outside the tile, or -1
+    uint32_t cl;       // control (or phase) mask over local tile bits
+    uint32_t aux;      // SW_BLOCK: offset of the block matrix in the block table
+    uint64_t co;       // control (or phase) mask over global bits outside the tile
+    double m[8];       // SW_DENSE1: m00, m01, m10, m11 (re, im); SW_PHASE: m[0..1]; SW_DIAG1: d0, d1
+};
+
+struct SwPhase {
+    uint32_t op_begin, op_end;
+    uint32_t reg_mask;   // local tile bits held in registers during this phase
+    uint32_t kind;       // 0: register phase, 1: fused dense block on LDS
+    uint8_t rpos[8];     // local positions of the register bits (ascending)
+    uint8_t tbit[16];    // local positions of the thread bits (ascending)
+};
+
+// A planned step of the gate stream.
+struct PlanStep {
+    bool sweep = false;
+    std::vector<int> tile;          // sweep: global bit positions of the tile (ascending)
+    std::vector<size_t> gates;      // indices into the queue, in application order
+};
+
+// Tile geometry per precision: M local bits, R register bits, L low bits kept
+// in every tile for coalesced 2^L-element runs.
+struct TileGeom {
+    int M, R, L;
+};
+TileGeom tile_geometry(Precision p);
+
+// Is a gate applicable inside a tile (targets that must be local, if any)?
+bool sweepable(const GateOp& op, std::vector<int>* local_targets);
+
+// Greedy plan: maximal runs of gates whose non-diagonal targets fit M tile
+// bits (the L low bits always included), respecting per-qubit order.
+std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g);
+
+// Apply the pending gates to columns [col_begin, col_end) of the state with
+// the planned schedule: runs of gates whose qubits fit one tile are applied
+// in one HBM sweep; `fusion_k` > 0 composes runs into dense blocks of at most
+// that many qubits first (complex64 path).
+void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size_t col_begin, size_t col_end,
+                 int fusion_k, bool tiling, RunStats& stats);
+
+}  // namespace q1t

@@ -1,0 +1,194 @@
+"""State-level Python wrapper of the GpuVectorState C ABI (include/q1tsim_gvs.h).
+
+``GpuVectorState`` exposes the QuState operations (src/qustate.rs:5-89) on
+the MI355X statevector; gates are given as FP64 matrices plus qubit indices,
+as the trait hands them to a backend.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, sizes
+
+PREC = {"c128": 0, "c64": 1}
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _u64ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+_matrix_cache = {}
+
+
+def gate_matrix(name, params=()):
+    """Gate::matrix() of a named gate from the library's gate table (FP64)."""
+    key = (name.lower(), tuple(float(p) for p in params))
+    m = _matrix_cache.get(key)
+    if m is None:
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        out = np.zeros(2 * 64 * 64, dtype=np.float64)
+        k = ctypes.c_size_t(0)
+        check(lib().q1t_gate_matrix(name.encode(), _dptr(p) if len(p) else None, len(p), _dptr(out), len(out),
+                                    ctypes.byref(k)))
+        d = 1 << k.value
+        m = out[:2 * d * d].view(np.complex128).reshape(d, d).copy()
+        if len(_matrix_cache) < 4096:
+            _matrix_cache[key] = m
+    return m
+
+
+def compare(a, b, col_a=0, col_b=0):
+    """(infidelity, max-abs difference) between one column of each state (FP64 on device)."""
+    out = np.zeros(5, dtype=np.float64)
+    check(lib().q1t_gvs_compare(a.h, col_a, b.h, col_b, _dptr(out)))
+    ov = complex(out[0], out[1])
+    fid = abs(ov) ** 2 / (out[3] * out[4])
+    return 1.0 - fid, float(out[2])
+
+
+class Xoshiro:
+    """The library's xoshiro256** stream (SplitMix64 seeding) as a q1t_rng."""
+
+    def __init__(self, seed):
+        self.h = lib().q1t_xoshiro_new(int(seed))
+        fn = ctypes.cast(lib().q1t_xoshiro_next, ctypes.c_void_p).value
+        self.rng = _lib.Rng(self.h, ctypes.CFUNCTYPE(ctypes.c_uint64, ctypes.c_void_p)(fn))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().q1t_xoshiro_free(self.h)
+            self.h = None
+
+
+class GpuVectorState:
+    def __init__(self, nr_bits, nr_shots, precision="c128"):
+        h = ctypes.c_void_p()
+        check(lib().q1t_gvs_new(nr_bits, nr_shots, PREC[precision], ctypes.byref(h)))
+        self.h = h
+        self.nr_bits, self.nr_shots = nr_bits, nr_shots
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().q1t_gvs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def from_qubit_coefs(cls, coefs, nr_shots, precision="c128"):
+        s = cls(len(coefs) // 2, nr_shots, precision)
+        c = np.ascontiguousarray(np.asarray(coefs, dtype=np.complex128))
+        check(lib().q1t_gvs_from_qubit_coefs(s.h, _dptr(c), len(c)))
+        return s
+
+    @classmethod
+    def from_amplitudes(cls, amps, nr_shots, precision="c128"):
+        a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128))
+        s = cls(int(np.log2(len(a))), nr_shots, precision)
+        check(lib().q1t_gvs_set_amplitudes(s.h, _dptr(a), len(a)))
+        return s
+
+    def set_random_normal(self, seed):
+        check(lib().q1t_gvs_set_random_normal(self.h, int(seed)))
+
+    def set_plan(self, fusion_k=0, tiling=True):
+        check(lib().q1t_gvs_set_plan(self.h, int(fusion_k), 1 if tiling else 0))
+
+    @property
+    def ncols(self):
+        return lib().q1t_gvs_nr_cols(self.h)
+
+    @property
+    def counts(self):
+        n = self.ncols
+        out = (ctypes.c_size_t * max(1, n))()
+        check(lib().q1t_gvs_counts(self.h, out, n))
+        return [int(out[i]) for i in range(n)]
+
+    def column(self, col):
+        out = np.zeros(1 << self.nr_bits, dtype=np.complex128)
+        check(lib().q1t_gvs_read_amplitudes(self.h, col, _dptr(out), len(out)))
+        return out
+
+    @property
+    def states(self):
+        return np.stack([self.column(c) for c in range(self.ncols)], axis=1)
+
+    def apply_gate(self, matrix, bits, desc="gate"):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+        k = int(np.log2(m.shape[0]))
+        check(lib().q1t_gvs_apply_gate(self.h, _dptr(m), k, sizes(bits), len(bits), desc.encode()))
+
+    def apply_unary_gate_all(self, matrix, desc="gate"):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+        check(lib().q1t_gvs_apply_unary_gate_all(self.h, _dptr(m), desc.encode()))
+
+    def apply_conditional_gate(self, control, matrix, bits, desc="gate"):
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+        k = int(np.log2(m.shape[0]))
+        c = np.ascontiguousarray(np.asarray(control, dtype=np.uint8))
+        check(lib().q1t_gvs_apply_conditional_gate(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(c),
+                                                   _dptr(m), k, sizes(bits), len(bits), desc.encode()))
+
+    def marginal_w0(self, qbit):
+        out = np.zeros(self.ncols, dtype=np.float64)
+        check(lib().q1t_gvs_marginal_w0(self.h, qbit, _dptr(out), len(out)))
+        return out
+
+    def measure_into(self, qbit, cbit, res, rng, collapse=True):
+        f = lib().q1t_gvs_measure_into if collapse else lib().q1t_gvs_peek_into
+        check(f(self.h, qbit, cbit, _u64ptr(res), len(res), ctypes.byref(rng.rng)))
+
+    def measure_with_n0(self, qbit, cbit, n0s, w0s, res, collapse=True):
+        n0 = sizes(n0s)
+        w0 = np.ascontiguousarray(np.asarray(w0s, dtype=np.float64))
+        check(lib().q1t_gvs_measure_with_n0(self.h, qbit, cbit, n0, _dptr(w0), len(w0s), _u64ptr(res), len(res),
+                                            1 if collapse else 0))
+
+    def measure_all_into(self, cbits, res, rng, collapse=True):
+        f = lib().q1t_gvs_measure_all_into if collapse else lib().q1t_gvs_peek_all_into
+        check(f(self.h, sizes(cbits), len(cbits), _u64ptr(res), len(res), ctypes.byref(rng.rng)))
+
+    def measure_all_uniforms(self, cbits, u01, res, collapse=True):
+        u = np.ascontiguousarray(np.asarray(u01, dtype=np.float64))
+        check(lib().q1t_gvs_measure_all_uniforms(self.h, sizes(cbits), len(cbits), _dptr(u), _u64ptr(res), len(res),
+                                                 1 if collapse else 0))
+
+    def reset(self, bit, rng):
+        check(lib().q1t_gvs_reset(self.h, bit, ctypes.byref(rng.rng)))
+
+    def reset_all(self):
+        check(lib().q1t_gvs_reset_all(self.h))
+
+    def flush(self):
+        check(lib().q1t_gvs_flush(self.h))
+
+    def stats(self):
+        s = _lib.Stats()
+        check(lib().q1t_gvs_last_stats(self.h, ctypes.byref(s)))
+        return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
+                "gates": s.gates}
+
+    def set_profiling(self, on=True):
+        check(lib().q1t_gvs_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_profile(self):
+        """{kernel name: (launches, total ms, algorithmic bytes)} from HIP events."""
+        n = ctypes.c_size_t(0)
+        check(lib().q1t_gvs_kernel_profile(self.h, None, 0, ctypes.byref(n)))
+        arr = (_lib.KernelProfile * max(1, n.value))()
+        check(lib().q1t_gvs_kernel_profile(self.h, ctypes.cast(arr, ctypes.c_void_p), n.value, ctypes.byref(n)))
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].bytes))
+                for i in range(n.value)}
+
+    def apply_named(self, name, bits, params=()):
+        self.apply_gate(gate_matrix(name, params), bits, name)

@@ -1,0 +1,310 @@
+"""Python front-end over libq1tsim.so, mirroring q1tsim's python/q1tsim.py.
+
+Same class names, method names, argument meaning and error behaviour as the
+reference wrapper (python/q1tsim.py:5-529), bound with ctypes instead of cffi
+(cffi is not installed here); every circuit runs on the MI355X statevector.
+Build-added methods (backend selection, seeds, amplitude read-out, run
+statistics) are marked as such.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Q1tError, lib, sizes, unpack_result
+
+
+class RefParam(object):
+    """Reference parameter (python/q1tsim.py:5-36): its value is read when the
+    circuit executes, so assign() changes the next (re)execution."""
+
+    def __init__(self, value):
+        self.__val = ctypes.c_double(value)
+
+    def __float__(self):
+        return self.__val.value
+
+    def assign(self, value):
+        self.__val.value = value
+
+    def pointer(self):
+        return ctypes.pointer(self.__val)
+
+
+def _params(values):
+    arr = (_lib.Parameter * max(1, len(values)))()
+    for i, v in enumerate(values):
+        if isinstance(v, RefParam):
+            arr[i].value = 0.0
+            arr[i].value_ptr = v.pointer()
+        else:
+            arr[i].value = float(v)
+            arr[i].value_ptr = None
+    return arr
+
+
+class Circuit(object):
+    """A quantum circuit (python/q1tsim.py:38-529) on the GPU statevector."""
+
+    def __init__(self, nr_qbits, nr_cbits=0):
+        self.__sim = lib()
+        self.__ptr = self.__sim.circuit_new(nr_qbits, nr_cbits)
+        self.__keep = []      # RefParam objects must outlive the circuit
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc_value, traceback):
+        self.close()
+
+    def close(self):
+        if self.__ptr is not None:
+            self.__sim.circuit_free(self.__ptr)
+            self.__ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nr_qbits(self):
+        return self.__sim.circuit_nr_qbits(self.__ptr)
+
+    def nr_cbits(self):
+        return self.__sim.circuit_nr_cbits(self.__ptr)
+
+    def cstate(self):
+        return unpack_result(self.__sim.circuit_cstate(self.__ptr))
+
+    def add_gate(self, name, qbits, params=None):
+        cname = bytes(name, "utf-8")
+        q = sizes(qbits)
+        if params is None:
+            return unpack_result(self.__sim.circuit_add_gate(self.__ptr, cname, q, len(qbits), None, 0))
+        self.__keep.extend(p for p in params if isinstance(p, RefParam))
+        return unpack_result(self.__sim.circuit_add_gate(self.__ptr, cname, q, len(qbits), _params(params),
+                                                         len(params)))
+
+    def add_conditional_gate(self, control, target, name, qbits, params=None):
+        """The bit at the position of the first index in control is the least
+        significant bit of the control word."""
+        cname = bytes(name, "utf-8")
+        c, q = sizes(control), sizes(qbits)
+        if params is None:
+            return unpack_result(self.__sim.circuit_add_conditional_gate(
+                self.__ptr, c, len(control), target, cname, q, len(qbits), None, 0))
+        self.__keep.extend(p for p in params if isinstance(p, RefParam))
+        return unpack_result(self.__sim.circuit_add_conditional_gate(
+            self.__ptr, c, len(control), target, cname, q, len(qbits), _params(params), len(params)))
+
+    # named gates (python/q1tsim.py:118-306)
+    def ch(self, control, target):
+        return self.add_gate("CH", [control, target])
+
+    def crx(self, theta, control, target):
+        return self.add_gate("CRX", [control, target], [theta])
+
+    def cry(self, theta, control, target):
+        return self.add_gate("CRY", [control, target], [theta])
+
+    def crz(self, lmb, control, target):
+        return self.add_gate("CRZ", [control, target], [lmb])
+
+    def cx(self, control, target):
+        return self.add_gate("CX", [control, target])
+
+    def cy(self, control, target):
+        return self.add_gate("CY", [control, target])
+
+    def cz(self, control, target):
+        return self.add_gate("CZ", [control, target])
+
+    def h(self, qbit):
+        return self.add_gate("H", [qbit])
+
+    def i(self, qbit):
+        return self.add_gate("I", [qbit])
+
+    def rx(self, theta, qbit):
+        return self.add_gate("RX", [qbit], [theta])
+
+    def ry(self, theta, qbit):
+        return self.add_gate("RY", [qbit], [theta])
+
+    def rz(self, lmb, qbit):
+        return self.add_gate("RZ", [qbit], [lmb])
+
+    def s(self, qbit):
+        return self.add_gate("S", [qbit])
+
+    def sdg(self, qbit):
+        return self.add_gate("Sdg", [qbit])
+
+    def swap(self, qbit0, qbit1):
+        return self.add_gate("Swap", [qbit0, qbit1])
+
+    def t(self, qbit):
+        return self.add_gate("T", [qbit])
+
+    def tdg(self, qbit):
+        return self.add_gate("Tdg", [qbit])
+
+    def u1(self, lmb, qbit):
+        return self.add_gate("U1", [qbit], [lmb])
+
+    def u2(self, phi, lmb, qbit):
+        return self.add_gate("U2", [qbit], [phi, lmb])
+
+    def u3(self, theta, phi, lmb, qbit):
+        return self.add_gate("U3", [qbit], [theta, phi, lmb])
+
+    def v(self, qbit):
+        return self.add_gate("V", [qbit])
+
+    def vdg(self, qbit):
+        return self.add_gate("Vdg", [qbit])
+
+    def x(self, qbit):
+        return self.add_gate("X", [qbit])
+
+    def y(self, qbit):
+        return self.add_gate("Y", [qbit])
+
+    def z(self, qbit):
+        return self.add_gate("Z", [qbit])
+
+    # build-added names (composite.rs:289-445 gate table)
+    def cu1(self, lmb, control, target):
+        return self.add_gate("CU1", [control, target], [lmb])
+
+    def ccx(self, c0, c1, target):
+        return self.add_gate("CCX", [c0, c1, target])
+
+    # measurement (python/q1tsim.py:308-452)
+    def __peek_measure_basis(self, qbit, cbit, basis, collapse):
+        return unpack_result(self.__sim.circuit_measure(self.__ptr, qbit, cbit, bytes(basis, "utf-8")[:1],
+                                                        1 if collapse else 0))
+
+    def __peek_measure_all_basis(self, cbits, basis, collapse):
+        return unpack_result(self.__sim.circuit_measure_all(self.__ptr, sizes(cbits), len(cbits),
+                                                            bytes(basis, "utf-8")[:1], 1 if collapse else 0))
+
+    def measure_basis(self, qbit, cbit, basis):
+        return self.__peek_measure_basis(qbit, cbit, basis, True)
+
+    def measure_x(self, qbit, cbit):
+        return self.measure_basis(qbit, cbit, "X")
+
+    def measure_y(self, qbit, cbit):
+        return self.measure_basis(qbit, cbit, "Y")
+
+    def measure_z(self, qbit, cbit):
+        return self.measure_basis(qbit, cbit, "Z")
+
+    def measure(self, qbit, cbit):
+        return self.measure_z(qbit, cbit)
+
+    def measure_all_basis(self, cbits, basis):
+        return self.__peek_measure_all_basis(cbits, basis, True)
+
+    def measure_all(self, cbits):
+        return self.measure_all_basis(cbits, "Z")
+
+    def peek_basis(self, qbit, cbit, basis):
+        return self.__peek_measure_basis(qbit, cbit, basis, False)
+
+    def peek_x(self, qbit, cbit):
+        return self.peek_basis(qbit, cbit, "X")
+
+    def peek_y(self, qbit, cbit):
+        return self.peek_basis(qbit, cbit, "Y")
+
+    def peek_z(self, qbit, cbit):
+        return self.peek_basis(qbit, cbit, "Z")
+
+    def peek(self, qbit, cbit):
+        return self.peek_z(qbit, cbit)
+
+    def peek_all_basis(self, cbits, basis):
+        return self.__peek_measure_all_basis(cbits, basis, False)
+
+    def peek_all(self, cbits):
+        return self.peek_all_basis(cbits, "Z")
+
+    def reset(self, qbit):
+        return unpack_result(self.__sim.circuit_reset(self.__ptr, qbit))
+
+    def reset_all(self):
+        return unpack_result(self.__sim.circuit_reset_all(self.__ptr))
+
+    def execute(self, nr_shots):
+        return unpack_result(self.__sim.circuit_execute(self.__ptr, nr_shots))
+
+    def reexecute(self):
+        return unpack_result(self.__sim.circuit_reexecute(self.__ptr))
+
+    def histogram(self):
+        """Histogram with binary string keys, cbit 0 as the last character."""
+        return unpack_result(self.__sim.circuit_histogram(self.__ptr))
+
+    def latex(self):
+        return unpack_result(self.__sim.circuit_latex(self.__ptr))
+
+    def open_qasm(self):
+        return unpack_result(self.__sim.circuit_open_qasm(self.__ptr))
+
+    def c_qasm(self):
+        return unpack_result(self.__sim.circuit_c_qasm(self.__ptr))
+
+    # ---- build-added ----
+    def set_backend(self, precision="c128", fusion_k=0, tiling=True):
+        prec = {"c128": 0, "c64": 1}[precision]
+        _lib.check(self.__sim.q1t_circuit_set_backend(self.__ptr, prec, int(fusion_k), 1 if tiling else 0))
+
+    def seed(self, seed):
+        _lib.check(self.__sim.q1t_circuit_seed(self.__ptr, int(seed)))
+
+    def set_initial_amplitudes(self, amps):
+        a = np.ascontiguousarray(np.asarray(amps, dtype=np.complex128))
+        _lib.check(self.__sim.q1t_circuit_set_initial_amplitudes(
+            self.__ptr, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(a)))
+
+    def set_initial_random(self, seed):
+        _lib.check(self.__sim.q1t_circuit_set_initial_random(self.__ptr, int(seed)))
+
+    def barrier(self, qbits):
+        _lib.check(self.__sim.q1t_circuit_barrier(self.__ptr, sizes(qbits), len(qbits)))
+
+    def nr_columns(self):
+        n = ctypes.c_size_t(0)
+        _lib.check(self.__sim.q1t_circuit_nr_columns(self.__ptr, ctypes.byref(n)))
+        return n.value
+
+    def counts(self):
+        n = self.nr_columns()
+        out = (ctypes.c_size_t * max(1, n))()
+        _lib.check(self.__sim.q1t_circuit_counts(self.__ptr, out, n))
+        return [int(out[i]) for i in range(n)]
+
+    def amplitudes(self, col=0):
+        n = self.nr_qbits()
+        out = np.zeros(1 << n, dtype=np.complex128)
+        _lib.check(self.__sim.q1t_circuit_read_amplitudes(
+            self.__ptr, col, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1 << n))
+        return out
+
+    def stats(self):
+        s = _lib.Stats()
+        _lib.check(self.__sim.q1t_circuit_last_stats(self.__ptr, ctypes.byref(s)))
+        return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
+                "gates": s.gates}
+
+    def histogram_vec(self):
+        n = 1 << self.nr_cbits()
+        out = (ctypes.c_size_t * n)()
+        _lib.check(self.__sim.q1t_circuit_histogram_vec(self.__ptr, out, n))
+        return [int(out[i]) for i in range(n)]
+
+
+__all__ = ["Circuit", "RefParam", "Q1tError"]

@@ -1,0 +1,120 @@
+"""GPU parity of the tiled multi-gate sweeps (the plain c128 path's HBM
+schedule) and of the per-gate kernels they fall back to."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+from oracle import oracle as orc  # noqa: E402
+from q1tsim_amd import workloads  # noqa: E402
+from q1tsim_amd.gvs import GpuVectorState  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def run(n, ops, amps, prec, tiling, ncols_ctrl=None):
+    g = GpuVectorState.from_amplitudes(amps, 4 if ncols_ctrl is not None else 1, prec)
+    g.set_plan(0, tiling)
+    if ncols_ctrl is not None:
+        g.apply_conditional_gate(ncols_ctrl, orc.gate_matrix("rx", [0.9]), [n - 1])
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    g.flush()
+    return g
+
+
+def oracle(n, ops, amps, ncols_ctrl=None):
+    r = orc.RefState(n, 4 if ncols_ctrl is not None else 1, amplitudes=amps)
+    if ncols_ctrl is not None:
+        r.apply_conditional_gate(ncols_ctrl, "rx", [n - 1], [0.9])
+    for name, bits, params in ops:
+        r.apply_gate(name, bits, params)
+    return r.states
+
+
+def rand_state(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return a / np.linalg.norm(a)
+
+
+def mixed_ops(n, count, seed):
+    """Every sweepable kind: dense one-qubit, controlled (controls in and out
+    of the tile), phases, diagonals on any bit, plus non-sweepable gates
+    (swap, dense two-qubit) that cut sweeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    names = ["h", "x", "y", "rx", "ry", "rz", "s", "t", "u3", "v", "z", "cx", "cz", "ch", "crx", "crz",
+             "cu1", "ccx", "ccz", "cy", "cu3", "ccrz", "swap"]
+    ops = []
+    for _ in range(count):
+        name = names[int(rng.integers(len(names)))]
+        kind, nctrl, npar = orc.GATES[name]
+        k = (2 if kind == orc.SWAP else 1) + nctrl
+        bits = [int(b) for b in rng.permutation(n)[:k]]
+        ops.append((name, bits, [float(v) for v in rng.uniform(0, 2 * np.pi, npar)]))
+    return ops
+
+
+@pytest.mark.parametrize("n", [14, 16, 19])
+def test_sweep_mixed_c128(n):
+    a = rand_state(n, n)
+    ops = mixed_ops(n, 300, n)
+    g = run(n, ops, a, "c128", True)
+    st = g.stats()
+    assert st["sweeps"] >= 1
+    ref = oracle(n, ops, a)
+    err = float(np.abs(g.states - ref).max())
+    assert err <= 1e-12, err
+
+
+def test_sweep_matches_per_gate_exactly_layered():
+    n = 18
+    ops = workloads.random_layered(n, 20, 1)
+    g1 = run(n, ops, rand_state(n, 1), "c128", True)
+    g0 = run(n, ops, rand_state(n, 1), "c128", False)
+    d = float(np.abs(g1.states - g0.states).max())
+    assert d <= 1e-13, d
+    st = g1.stats()
+    print("layered n=18: %d gates in %d sweeps, %d launches" % (st["gates"], st["sweeps"], st["launches"]))
+    assert st["sweeps"] < 40
+
+
+def test_sweep_multicolumn():
+    n = 15
+    a = rand_state(n, 3)
+    ops = mixed_ops(n, 120, 3)
+    ctrl = [1, 0, 0, 1]
+    g = run(n, ops, a, "c128", True, ncols_ctrl=ctrl)
+    ref = oracle(n, ops, a, ncols_ctrl=ctrl)
+    assert g.counts == [1, 2, 1]
+    err = float(np.abs(g.states - ref).max())
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("n", [15, 20])
+def test_sweep_c64(n):
+    a = rand_state(n, 7)
+    ops = mixed_ops(n, 200, 7)
+    g = run(n, ops, a, "c64", True)
+    ref = oracle(n, ops, a)
+    col = g.states[:, 0]
+    err = float(np.abs(col - ref[:, 0]).max())
+    fid = abs(np.vdot(ref[:, 0], col)) ** 2
+    assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+
+
+def test_qft_sweeps():
+    n = 20
+    g = GpuVectorState(n, 1)
+    g.set_plan(0, True)
+    g.set_random_normal(3)
+    a = g.column(0)
+    for name, bits, params in workloads.qft(n):
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    out = g.column(0)
+    err = float(np.abs(out - np.sqrt(1 << n) * np.fft.ifft(a)).max())
+    assert err < 1e-11, err
