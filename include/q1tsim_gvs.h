@@ -66,6 +66,7 @@ typedef struct {
     uint64_t launches;     /* kernels launched */
     uint64_t sweeps;       /* HBM sweeps over the state */
     uint64_t gates;        /* gates applied */
+    uint64_t blocks;       /* fused dense blocks applied (per column) */
 } q1t_stats;
 
 const char* q1t_last_error(void);

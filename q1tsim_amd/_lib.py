@@ -27,7 +27,7 @@ class Result(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("gate_seconds", ctypes.c_double), ("bytes", ctypes.c_double), ("launches", ctypes.c_uint64),
-                ("sweeps", ctypes.c_uint64), ("gates", ctypes.c_uint64)]
+                ("sweeps", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("blocks", ctypes.c_uint64)]
 
 
 class KernelProfile(ctypes.Structure):

@@ -323,7 +323,7 @@ extern "C" int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out)
     NEED(ptr);
     NEED(out);
     const RunStats& r = ptr->c.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks};
     return 0;
 }
 
@@ -574,7 +574,7 @@ extern "C" int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out)
     NEED(s);
     NEED(out);
     const RunStats& r = s->s.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks};
     return 0;
 }
 

@@ -20,6 +20,7 @@ struct RunStats {
     uint64_t launches = 0;        // kernels launched
     uint64_t sweeps = 0;          // full-state passes
     uint64_t gates = 0;           // gates applied
+    uint64_t blocks = 0;          // fused dense blocks applied per column
 };
 
 // Kernel time measured with HIP events on the state's stream.

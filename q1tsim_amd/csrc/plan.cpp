@@ -1,0 +1,412 @@
+// Gate-stream planning for the tile sweeps (host side).
+//
+// plan_gates   : splits the gate queue into HBM sweeps (runs of gates whose
+//                qubits fit one tile) and single-gate kernels.
+// build_program: per sweep, register phases for per-gate arithmetic.
+// build_fused_program: per sweep, dense k-qubit blocks composed in FP64 and
+//                rounded once to FP32 (applied by MFMA), plus register phases
+//                for the diagonal gates that reach outside the tile.
+#include <algorithm>
+#include <cstring>
+#include <set>
+
+#include "sweep.h"
+
+namespace q1t {
+
+TileGeom tile_geometry(Precision p)
+{
+    // 64 KiB of LDS per tile either way (two workgroups per CU); 16 amplitudes
+    // per thread for complex128, 32 for complex64; 256 threads either way.
+    if (p == PREC_C128) return TileGeom{12, 4, 4};
+    return TileGeom{13, 5, 5};
+}
+
+static std::vector<int> mask_list(uint64_t m)
+{
+    std::vector<int> r;
+    for (int b = 0; b < 64; ++b)
+        if ((m >> b) & 1) r.push_back(b);
+    return r;
+}
+
+static uint64_t touched_bits(const GateOp& op)
+{
+    uint64_t m = op.ctrl_mask;
+    for (int t : op.targets) m |= uint64_t(1) << t;
+    return m;
+}
+
+static bool is_diagonal(const GateOp& op) { return op.kind == OpKind::Diag || op.kind == OpKind::PhaseOnes; }
+
+bool sweepable(const GateOp& op, std::vector<int>* local_targets)
+{
+    if (local_targets) local_targets->clear();
+    switch (op.kind) {
+    case OpKind::Identity:
+    case OpKind::PhaseOnes:
+        return true;
+    case OpKind::Diag:
+        return op.targets.size() == 1;
+    case OpKind::Dense:
+        if (op.targets.size() != 1) return false;
+        if (local_targets) local_targets->push_back(op.targets[0]);
+        return true;
+    default:
+        return false;
+    }
+}
+
+// Which bits a gate needs inside the tile, or false if it cannot join a sweep.
+// Plain: non-diagonal targets (controls and diagonal factors outside the tile
+// are per-tile predicates / constants).  Fused (fuse_k > 0): every bit of a
+// non-diagonal gate, so it can be absorbed into a block; diagonal one-target
+// gates and phases may still reach outside.
+static bool tile_need(const GateOp& op, int fuse_k, uint64_t* need)
+{
+    *need = 0;
+    if (fuse_k > 0) {
+        const uint64_t bits = touched_bits(op);
+        if (__builtin_popcountll(bits) > fuse_k) return false;
+        if (op.kind == OpKind::PhaseOnes || (op.kind == OpKind::Diag && op.targets.size() == 1)) return true;
+        *need = bits;
+        return true;
+    }
+    std::vector<int> lt;
+    if (!sweepable(op, &lt)) return false;
+    for (int t : lt) *need |= uint64_t(1) << t;
+    return true;
+}
+
+std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k)
+{
+    std::vector<PlanStep> steps;
+    std::vector<size_t> rem;
+    for (size_t i = 0; i < q.size(); ++i)
+        if (q[i].op.kind != OpKind::Identity) rem.push_back(i);
+    const bool tiles_ok = n >= g.M + 2;
+    while (!rem.empty()) {
+        if (!tiles_ok) {
+            for (size_t i : rem) steps.push_back(PlanStep{false, {}, {i}});
+            break;
+        }
+        uint64_t tile = (uint64_t(1) << g.L) - 1;
+        int tcount = g.L;
+        uint64_t blocked = 0;
+        std::vector<size_t> sel;
+        for (size_t i : rem) {
+            const GateOp& op = q[i].op;
+            const uint64_t bits = touched_bits(op);
+            uint64_t need = 0;
+            if ((bits & blocked) || !tile_need(op, fuse_k, &need)) {
+                blocked |= bits;
+                continue;
+            }
+            need &= ~tile;
+            const int nn = __builtin_popcountll(need);
+            if (tcount + nn <= g.M) {
+                tile |= need;
+                tcount += nn;
+                sel.push_back(i);
+            } else {
+                blocked |= bits;
+            }
+        }
+        if (sel.empty()) {
+            // the first remaining gate cannot run inside a tile
+            steps.push_back(PlanStep{false, {}, {rem.front()}});
+        } else if (sel.size() == 1 && is_diagonal(q[sel[0]].op)) {
+            // a lone diagonal gate touches only part of the state: its own kernel is cheaper
+            steps.push_back(PlanStep{false, {}, sel});
+        } else {
+            // fill the tile up to M bits with the lowest unused bits
+            for (int b = 0; b < n && tcount < g.M; ++b)
+                if (!((tile >> b) & 1)) {
+                    tile |= uint64_t(1) << b;
+                    ++tcount;
+                }
+            PlanStep st;
+            st.sweep = true;
+            st.tile = mask_list(tile);
+            st.gates = sel;
+            steps.push_back(st);
+        }
+        // drop the consumed gates, keeping the original order of the rest
+        const std::set<size_t> done(steps.back().gates.begin(), steps.back().gates.end());
+        std::vector<size_t> next;
+        for (size_t i : rem)
+            if (!done.count(i)) next.push_back(i);
+        rem.swap(next);
+    }
+    return steps;
+}
+
+namespace {
+
+struct LocalMap {
+    int local_of[64];
+    explicit LocalMap(const std::vector<int>& tile)
+    {
+        for (int b = 0; b < 64; ++b) local_of[b] = -1;
+        for (size_t j = 0; j < tile.size(); ++j) local_of[tile[j]] = int(j);
+    }
+    void split(uint64_t m, uint32_t& cl, uint64_t& co) const
+    {
+        cl = 0;
+        co = 0;
+        for (int b = 0; b < 64; ++b)
+            if ((m >> b) & 1) {
+                if (local_of[b] >= 0) cl |= 1u << local_of[b];
+                else co |= uint64_t(1) << b;
+            }
+    }
+};
+
+SwPhase make_phase(uint32_t kind, uint32_t regs, int M, uint32_t op_begin)
+{
+    SwPhase ph;
+    std::memset(&ph, 0, sizeof(ph));
+    ph.kind = kind;
+    ph.reg_mask = regs;
+    ph.op_begin = op_begin;
+    int nr = 0, nt = 0;
+    for (int b = 0; b < M; ++b) {
+        if ((regs >> b) & 1) ph.rpos[nr++] = uint8_t(b);
+        else ph.tbit[nt++] = uint8_t(b);
+    }
+    return ph;
+}
+
+SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
+{
+    SwOp o;
+    std::memset(&o, 0, sizeof(o));
+    o.dlocal = -1;
+    o.dglobal = -1;
+    lm.split(op.ctrl_mask, o.cl, o.co);
+    if (op.kind == OpKind::Dense) {
+        o.type = SW_DENSE1;
+        const int lb = lm.local_of[op.targets[0]];
+        int idx = 0;
+        for (int b = 0; b < lb; ++b) idx += (regs >> b) & 1;
+        o.reg = idx;
+        for (int k = 0; k < 4; ++k) {
+            o.m[2 * k] = op.m[k].real();
+            o.m[2 * k + 1] = op.m[k].imag();
+        }
+    } else if (op.kind == OpKind::PhaseOnes) {
+        o.type = SW_PHASE;
+        o.m[0] = op.m[0].real();
+        o.m[1] = op.m[0].imag();
+    } else {  // Diag on one target
+        o.type = SW_DIAG1;
+        const int t = op.targets[0];
+        if (lm.local_of[t] >= 0) o.dlocal = lm.local_of[t];
+        else o.dglobal = t;
+        o.m[0] = op.m[0].real();
+        o.m[1] = op.m[0].imag();
+        o.m[2] = op.m[1].real();
+        o.m[3] = op.m[1].imag();
+    }
+    return o;
+}
+
+// One register phase by list scheduling over `pending`: a gate waits for
+// every earlier gate that shares a bit; one-qubit updates on up to R local
+// bits share the phase.  Returns the gates left for later phases.
+std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::vector<size_t>& pending,
+                                  const LocalMap& lm, int M, int R, std::vector<SwPhase>& phases,
+                                  std::vector<SwOp>& ops)
+{
+    uint32_t regs = 0;
+    int nregs = 0;
+    uint64_t blocked = 0;
+    std::vector<size_t> chosen, rest;
+    for (size_t i : pending) {
+        const GateOp& op = q[i].op;
+        const uint64_t bits = touched_bits(op);
+        if ((bits & blocked) || !sweepable(op, nullptr)) {
+            blocked |= bits;
+            rest.push_back(i);
+            continue;
+        }
+        if (op.kind == OpKind::Dense) {
+            const int lb = lm.local_of[op.targets[0]];
+            if ((regs >> lb) & 1) {
+                chosen.push_back(i);
+            } else if (nregs < R) {
+                regs |= 1u << lb;
+                ++nregs;
+                chosen.push_back(i);
+            } else {
+                blocked |= bits;
+                rest.push_back(i);
+            }
+        } else {
+            chosen.push_back(i);
+        }
+    }
+    for (int b = 0; b < M && nregs < R; ++b)
+        if (!((regs >> b) & 1)) {
+            regs |= 1u << b;
+            ++nregs;
+        }
+    SwPhase ph = make_phase(0, regs, M, uint32_t(ops.size()));
+    for (size_t i : chosen) ops.push_back(make_op(q[i].op, lm, regs));
+    ph.op_end = uint32_t(ops.size());
+    phases.push_back(ph);
+    return rest;
+}
+
+// Apply a gate to every column of the block matrix U (dimension 2^|B|);
+// pos[b] = block-index bit of global bit b.
+void compose(std::vector<cdouble>& U, int nb, const GateOp& op, const int* pos)
+{
+    const size_t d = size_t(1) << nb;
+    uint64_t cmask = 0;
+    for (int b = 0; b < 64; ++b)
+        if ((op.ctrl_mask >> b) & 1) cmask |= uint64_t(1) << pos[b];
+    std::vector<int> tb;
+    for (int t : op.targets) tb.push_back(pos[t]);
+    const int kt = int(tb.size());
+    std::vector<cdouble> tmp(size_t(1) << kt), out(size_t(1) << kt);
+    for (size_t c = 0; c < d; ++c) {
+        cdouble* col = nullptr;  // column c is strided: U[r * d + c]
+        (void)col;
+        switch (op.kind) {
+        case OpKind::Identity:
+            break;
+        case OpKind::PhaseOnes:
+            for (size_t r = 0; r < d; ++r)
+                if ((r & cmask) == cmask) U[r * d + c] *= op.m[0];
+            break;
+        case OpKind::Diag:
+            for (size_t r = 0; r < d; ++r) {
+                if ((r & cmask) != cmask) continue;
+                size_t l = 0;
+                for (int q = 0; q < kt; ++q) l = (l << 1) | ((r >> tb[q]) & 1);
+                U[r * d + c] *= op.m[l];
+            }
+            break;
+        case OpKind::Swap2: {
+            const size_t b0 = size_t(1) << tb[0], b1 = size_t(1) << tb[1];
+            for (size_t r = 0; r < d; ++r)
+                if ((r & cmask) == cmask && (r & b0) && !(r & b1)) std::swap(U[r * d + c], U[(r ^ b0 ^ b1) * d + c]);
+            break;
+        }
+        case OpKind::Dense: {
+            size_t tmask = 0;
+            for (int t : tb) tmask |= size_t(1) << t;
+            const size_t D = size_t(1) << kt;
+            for (size_t r = 0; r < d; ++r) {
+                if ((r & tmask) || (r & cmask) != cmask) continue;
+                for (size_t l = 0; l < D; ++l) {
+                    size_t ri = r;
+                    for (int q = 0; q < kt; ++q)
+                        if ((l >> (kt - 1 - q)) & 1) ri |= size_t(1) << tb[q];
+                    tmp[l] = U[ri * d + c];
+                }
+                for (size_t l = 0; l < D; ++l) {
+                    cdouble acc = 0;
+                    for (size_t m = 0; m < D; ++m) acc += op.m[l * D + m] * tmp[m];
+                    out[l] = acc;
+                }
+                for (size_t l = 0; l < D; ++l) {
+                    size_t ri = r;
+                    for (int q = 0; q < kt; ++q)
+                        if ((l >> (kt - 1 - q)) & 1) ri |= size_t(1) << tb[q];
+                    U[ri * d + c] = out[l];
+                }
+            }
+            break;
+        }
+        }
+    }
+}
+
+}  // namespace
+
+void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops)
+{
+    const LocalMap lm(step.tile);
+    std::vector<size_t> pending = step.gates;
+    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops);
+}
+
+void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
+                         std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
+                         int* nblocks)
+{
+    const LocalMap lm(step.tile);
+    std::vector<size_t> pending = step.gates;
+    while (!pending.empty()) {
+        // grow one block: gates whose bits are all in the tile, while the
+        // union of their local bits stays within K (order kept per bit)
+        std::vector<int> B;            // local bits, B[0] = block MSB
+        uint64_t blocked = 0;
+        std::vector<size_t> absorbed, rest;
+        for (size_t i : pending) {
+            const GateOp& op = q[i].op;
+            const uint64_t bits = touched_bits(op);
+            bool ok = !(bits & blocked);
+            std::vector<int> add;
+            if (ok) {
+                for (int b = 0; b < 64 && ok; ++b)
+                    if ((bits >> b) & 1) {
+                        const int lb = lm.local_of[b];
+                        if (lb < 0) ok = false;
+                        else if (std::find(B.begin(), B.end(), lb) == B.end()) add.push_back(lb);
+                    }
+                ok = ok && int(B.size() + add.size()) <= K;
+            }
+            if (!ok) {
+                blocked |= bits;
+                rest.push_back(i);
+                continue;
+            }
+            B.insert(B.end(), add.begin(), add.end());
+            absorbed.push_back(i);
+        }
+        if (absorbed.empty()) {
+            pending = one_reg_phase(q, pending, lm, M, R, phases, ops);
+            continue;
+        }
+        // pad to at least 3 bits (the MFMA tile needs 2D >= 16)
+        for (int b = 0; b < M && B.size() < 3; ++b)
+            if (std::find(B.begin(), B.end(), b) == B.end()) B.push_back(b);
+        const int nb = int(B.size());
+        const size_t d = size_t(1) << nb;
+        int pos[64];
+        for (int b = 0; b < 64; ++b) pos[b] = -1;
+        for (int j = 0; j < nb; ++j) pos[step.tile[B[j]]] = nb - 1 - j;
+        std::vector<cdouble> U(d * d, 0.0);
+        for (size_t r = 0; r < d; ++r) U[r * d + r] = 1.0;
+        for (size_t i : absorbed) compose(U, nb, q[i].op, pos);
+        // real embedding [[Ur, -Ui], [Ui, Ur]], rounded once to FP32
+        const uint32_t off = uint32_t(blocks.size());
+        const size_t d2 = 2 * d;
+        blocks.resize(blocks.size() + d2 * d2);
+        float* A = blocks.data() + off;
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c) {
+                const cdouble u = U[r * d + c];
+                A[r * d2 + c] = float(u.real());
+                A[r * d2 + d + c] = float(-u.imag());
+                A[(d + r) * d2 + c] = float(u.imag());
+                A[(d + r) * d2 + d + c] = float(u.real());
+            }
+        uint32_t regs = 0;
+        for (int lb : B) regs |= 1u << lb;
+        SwPhase ph = make_phase(1, regs, M, off);
+        // block bits MSB first; column bits ascending
+        for (int j = 0; j < nb; ++j) ph.rpos[j] = uint8_t(B[j]);
+        ph.reg_mask = uint32_t(nb);
+        ph.op_end = off;
+        phases.push_back(ph);
+        if (nblocks) ++*nblocks;
+        pending.swap(rest);
+    }
+}
+
+}  // namespace q1t

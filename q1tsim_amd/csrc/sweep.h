@@ -48,8 +48,20 @@ TileGeom tile_geometry(Precision p);
 bool sweepable(const GateOp& op, std::vector<int>* local_targets);
 
 // Greedy plan: maximal runs of gates whose non-diagonal targets fit M tile
-// bits (the L low bits always included), respecting per-qubit order.
-std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g);
+// bits (the L low bits always included), respecting per-qubit order.  With
+// fuse_k > 0 the non-diagonal gates need all their bits in the tile.
+std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k);
+
+// Register phases of one sweep (per-gate arithmetic).
+void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops);
+
+// Fused program of one sweep: dense blocks of at most K local qubits
+// (composed in FP64, real-embedded, rounded to FP32 into `blocks`) and
+// register phases for the rest.
+void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
+                         std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
+                         int* nblocks);
 
 // Apply the pending gates to columns [col_begin, col_end) of the state with
 // the planned schedule: runs of gates whose qubits fit one tile are applied

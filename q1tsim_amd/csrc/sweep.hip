@@ -28,19 +28,13 @@ namespace q1t {
                         std::string("HIP error in " #expr ": ") + hipGetErrorString(e_));          \
     } while (0)
 
-TileGeom tile_geometry(Precision p)
-{
-    // 64 KiB of LDS per tile either way: two workgroups per CU.
-    if (p == PREC_C128) return TileGeom{12, 4, 4};
-    return TileGeom{13, 5, 5};
-}
-
 struct SweepArgs {
     uint64_t ntiles;
     PosList tset;        // global positions of the tile bits (ascending)
     int tpos[16];        // tpos[j] = global position of local bit j
     const SwPhase* phases;
     const SwOp* ops;
+    const float* blocks;   // real-embedded block matrices (fused segments)
     int nphases;
 };
 
@@ -69,8 +63,74 @@ __device__ __forceinline__ void dense_pairs(V (&v)[NR], const uint32_t (&lidx)[N
     }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Dense K-qubit block Y = U X on the complex64 tile in LDS with FP32 MFMA
+// (v_mfma_f32_16x16x4f32, FP32 accumulation).  X is the tile viewed as a
+// D x C matrix (rows: the K block bits, columns: the other M-K local bits);
+// the complex product is the real GEMM [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi]
+// with the 2D x 2D real matrix A in global memory.  Each wave owns 16-column
+// chunks, so the in-place update needs no barrier until the block is done.
+// Operand layouts: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16(i/4), element i%4.
+template <int K, int M, int SW>
+__device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict__ A, const SwPhase& ph, int tid)
+{
+    constexpr int D = 1 << K, D2 = 2 * D, C = 1 << (M - K);
+    constexpr int NS = D2 / 4, NRT = D2 / 16;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = int(blockDim.x) >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    float a[NRT][NS];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) a[rt][s] = A[(16 * rt + li) * D2 + 4 * s + lg];
+    uint32_t rbit[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) rbit[j] = 1u << ph.rpos[j];
+    // local index bits of block row r (row bit K-1-j is local bit rpos[j])
+    auto rowpart = [&](int r) {
+        uint32_t p = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if ((r >> (K - 1 - j)) & 1) p |= rbit[j];
+        return p;
+    };
+    for (int chunk = wave; chunk < C / 16; chunk += nwaves) {
+        const int c = chunk * 16 + li;
+        uint32_t cp = 0;
+#pragma unroll
+        for (int b = 0; b < M - K; ++b)
+            if ((c >> b) & 1) cp |= 1u << ph.tbit[b];
+        float bv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int rr = 4 * s + lg;
+            const float2 v = lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))];
+            bv[s] = rr < D ? v.x : v.y;
+        }
+        f32x4 acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) {
+            acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], bv[s], acc[rt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = 16 * rt + 4 * lg + v;
+                float* dst = reinterpret_cast<float*>(&lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))]);
+                dst[rr < D ? 0 : 1] = acc[rt][v];
+            }
+    }
+}
+
 template <typename V, int M, int R, int SW>
-__global__ void __launch_bounds__(1 << (M - R)) k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
+__global__ void __launch_bounds__(1 << (M - R))
+__attribute__((amdgpu_waves_per_eu(2)))
+k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 {
     constexpr int NT = 1 << (M - R), NR = 1 << R;
     using Rl = typename RealOf<V>::type;
@@ -101,6 +161,17 @@ __global__ void __launch_bounds__(1 << (M - R)) k_sweep(V* __restrict__ amp, uin
 
         for (int p = 0; p < sa.nphases; ++p) {
             const SwPhase& ph = sa.phases[p];
+            if constexpr (sizeof(V) == 8) {
+                if (ph.kind == 1) {
+                    // fused dense block: K = reg_mask, matrix at blocks + op_begin
+                    const float* A = sa.blocks + ph.op_begin;
+                    // the planner emits K = 3 or 4 only
+                    if (ph.reg_mask == 3) block_mfma<3, M, SW>(reinterpret_cast<float2*>(lds), A, ph, tid);
+                    else block_mfma<4, M, SW>(reinterpret_cast<float2*>(lds), A, ph, tid);
+                    __syncthreads();
+                    continue;
+                }
+            }
             // register bits ph.rpos[0..R-1], thread bits ph.tbit[0..M-R-1]
             uint32_t tpart = 0;
 #pragma unroll
@@ -165,226 +236,9 @@ __global__ void __launch_bounds__(1 << (M - R)) k_sweep(V* __restrict__ amp, uin
     }
 }
 
-// ------------------------------------------------------------- planning
-
-static std::vector<int> mask_list(uint64_t m)
-{
-    std::vector<int> r;
-    for (int b = 0; b < 64; ++b)
-        if ((m >> b) & 1) r.push_back(b);
-    return r;
-}
-
-bool sweepable(const GateOp& op, std::vector<int>* local_targets)
-{
-    if (local_targets) local_targets->clear();
-    switch (op.kind) {
-    case OpKind::Identity:
-    case OpKind::PhaseOnes:
-        return true;
-    case OpKind::Diag:
-        return op.targets.size() == 1;
-    case OpKind::Dense:
-        if (op.targets.size() != 1) return false;
-        if (local_targets) local_targets->push_back(op.targets[0]);
-        return true;
-    default:
-        return false;
-    }
-}
-
-static uint64_t touched_bits(const GateOp& op)
-{
-    uint64_t m = op.ctrl_mask;
-    for (int t : op.targets) m |= uint64_t(1) << t;
-    return m;
-}
-
-std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g)
-{
-    std::vector<PlanStep> steps;
-    std::vector<size_t> rem;
-    for (size_t i = 0; i < q.size(); ++i)
-        if (q[i].op.kind != OpKind::Identity) rem.push_back(i);
-    const bool tiles_ok = n >= g.M + 2;
-    std::vector<int> lt;
-    while (!rem.empty()) {
-        if (!tiles_ok) {
-            for (size_t i : rem) steps.push_back(PlanStep{false, {}, {i}});
-            break;
-        }
-        uint64_t tile = (uint64_t(1) << g.L) - 1;
-        int tcount = g.L;
-        uint64_t blocked = 0;
-        std::vector<size_t> sel, keep;
-        for (size_t i : rem) {
-            const GateOp& op = q[i].op;
-            const uint64_t bits = touched_bits(op);
-            if (bits & blocked) {
-                blocked |= bits;
-                keep.push_back(i);
-                continue;
-            }
-            if (!sweepable(op, &lt)) {
-                blocked |= bits;
-                keep.push_back(i);
-                continue;
-            }
-            uint64_t need = 0;
-            for (int t : lt)
-                if (!((tile >> t) & 1)) need |= uint64_t(1) << t;
-            const int nn = __builtin_popcountll(need);
-            if (tcount + nn <= g.M) {
-                tile |= need;
-                tcount += nn;
-                sel.push_back(i);
-            } else {
-                blocked |= bits;
-                keep.push_back(i);
-            }
-        }
-        if (sel.empty()) {
-            // the first remaining gate cannot run inside a tile
-            steps.push_back(PlanStep{false, {}, {rem.front()}});
-        } else if (sel.size() == 1 && q[sel[0]].op.kind != OpKind::Dense) {
-            // a lone diagonal gate touches only part of the state: its own kernel is cheaper
-            steps.push_back(PlanStep{false, {}, sel});
-        } else {
-            // fill the tile up to M bits with the lowest unused bits
-            for (int b = 0; b < n && tcount < g.M; ++b)
-                if (!((tile >> b) & 1)) {
-                    tile |= uint64_t(1) << b;
-                    ++tcount;
-                }
-            PlanStep st;
-            st.sweep = true;
-            st.tile = mask_list(tile);
-            st.gates = sel;
-            steps.push_back(st);
-        }
-        // drop the consumed gates, keeping the original order of the rest
-        const std::set<size_t> done(steps.back().gates.begin(), steps.back().gates.end());
-        std::vector<size_t> next;
-        for (size_t i : rem)
-            if (!done.count(i)) next.push_back(i);
-        rem.swap(next);
-    }
-    return steps;
-}
-
-// Build the register phases of one sweep: list scheduling that keeps the
-// per-qubit order (a gate waits for every earlier gate that shares a bit)
-// and packs one-qubit updates on up to R local bits into each phase.
-static void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops)
-{
-    int local_of[64];
-    for (int b = 0; b < 64; ++b) local_of[b] = -1;
-    uint64_t tilemask = 0;
-    for (size_t j = 0; j < step.tile.size(); ++j) {
-        local_of[step.tile[j]] = int(j);
-        tilemask |= uint64_t(1) << step.tile[j];
-    }
-    auto split = [&](uint64_t m, uint32_t& cl, uint64_t& co) {
-        cl = 0;
-        co = 0;
-        for (int b = 0; b < 64; ++b)
-            if ((m >> b) & 1) {
-                if (local_of[b] >= 0) cl |= 1u << local_of[b];
-                else co |= uint64_t(1) << b;
-            }
-    };
-    std::vector<size_t> pending = step.gates;
-    while (!pending.empty()) {
-        uint32_t regs = 0;
-        int nregs = 0;
-        uint64_t blocked = 0;
-        std::vector<size_t> chosen, rest;
-        for (size_t i : pending) {
-            const GateOp& op = q[i].op;
-            const uint64_t bits = touched_bits(op);
-            if (bits & blocked) {
-                blocked |= bits;
-                rest.push_back(i);
-                continue;
-            }
-            if (op.kind == OpKind::Dense) {
-                const int lb = local_of[op.targets[0]];
-                if ((regs >> lb) & 1) {
-                    chosen.push_back(i);
-                } else if (nregs < R) {
-                    regs |= 1u << lb;
-                    ++nregs;
-                    chosen.push_back(i);
-                } else {
-                    blocked |= bits;
-                    rest.push_back(i);
-                }
-            } else {
-                chosen.push_back(i);
-            }
-        }
-        for (int b = 0; b < M && nregs < R; ++b)
-            if (!((regs >> b) & 1)) {
-                regs |= 1u << b;
-                ++nregs;
-            }
-        SwPhase ph;
-        std::memset(&ph, 0, sizeof(ph));
-        ph.op_begin = uint32_t(ops.size());
-        ph.reg_mask = regs;
-        {
-            int nr = 0, nt = 0;
-            for (int b = 0; b < M; ++b) {
-                if ((regs >> b) & 1) ph.rpos[nr++] = uint8_t(b);
-                else ph.tbit[nt++] = uint8_t(b);
-            }
-        }
-        for (size_t i : chosen) {
-            const GateOp& op = q[i].op;
-            SwOp o;
-            std::memset(&o, 0, sizeof(o));
-            o.dlocal = -1;
-            o.dglobal = -1;
-            if (op.kind == OpKind::Dense) {
-                o.type = SW_DENSE1;
-                const int lb = local_of[op.targets[0]];
-                int idx = 0;
-                for (int b = 0; b < lb; ++b) idx += (regs >> b) & 1;
-                o.reg = idx;
-                split(op.ctrl_mask, o.cl, o.co);
-                for (int k = 0; k < 4; ++k) {
-                    o.m[2 * k] = op.m[k].real();
-                    o.m[2 * k + 1] = op.m[k].imag();
-                }
-            } else if (op.kind == OpKind::PhaseOnes) {
-                o.type = SW_PHASE;
-                split(op.ctrl_mask, o.cl, o.co);
-                o.m[0] = op.m[0].real();
-                o.m[1] = op.m[0].imag();
-            } else {  // Diag on one target
-                o.type = SW_DIAG1;
-                split(op.ctrl_mask, o.cl, o.co);
-                const int t = op.targets[0];
-                if (local_of[t] >= 0) o.dlocal = local_of[t];
-                else o.dglobal = t;
-                o.m[0] = op.m[0].real();
-                o.m[1] = op.m[0].imag();
-                o.m[2] = op.m[1].real();
-                o.m[3] = op.m[1].imag();
-            }
-            ops.push_back(o);
-        }
-        ph.op_end = uint32_t(ops.size());
-        phases.push_back(ph);
-        pending.swap(rest);
-    }
-    (void)tilemask;
-}
-
 template <typename V, int M, int R, int SW>
 static void launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
-                         const SwOp* dops, size_t cb, size_t ce)
+                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce)
 {
     const int n = int(st.nr_bits());
     SweepArgs sa{};
@@ -396,6 +250,7 @@ static void launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase
     }
     sa.phases = dphases;
     sa.ops = dops;
+    sa.blocks = dblocks;
     sa.nphases = nphases;
     const uint64_t dim = uint64_t(1) << n;
     const unsigned gx = unsigned(std::min<uint64_t>(sa.ntiles, 2048));
@@ -412,27 +267,34 @@ static void launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase
 void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size_t col_begin, size_t col_end,
                  int fusion_k, bool tiling, RunStats& stats)
 {
-    (void)fusion_k;
     const int n = int(st.nr_bits());
     const TileGeom g = tile_geometry(st.precision());
     if (!tiling) {
         for (const PendingGate& pg : queue) st.apply_op(pg.op, col_begin, col_end);
         return;
     }
-    const std::vector<PlanStep> steps = plan_gates(queue, n, g);
-    // all sweep programs of this flush in one upload: [phases][ops] per sweep
+    // dense-block fusion runs on the complex64 state (FP32 MFMA)
+    // blocks of 3 or 4 local qubits: the MFMA stage is instantiated for those
+    const int K = (st.precision() == PREC_C64 && fusion_k > 0) ? std::min(4, std::max(3, fusion_k)) : 0;
+    const std::vector<PlanStep> steps = plan_gates(queue, n, g, K);
+    // all sweep programs of this flush in one upload: [phases][ops] per sweep, then the block table
     std::vector<std::vector<SwPhase>> phases(steps.size());
     std::vector<std::vector<SwOp>> ops(steps.size());
+    std::vector<float> blocks;
     std::vector<size_t> poff(steps.size()), ooff(steps.size());
     size_t total = 0;
+    int nblocks = 0;
     for (size_t s = 0; s < steps.size(); ++s) {
         if (!steps[s].sweep) continue;
-        build_program(queue, steps[s], g.M, g.R, phases[s], ops[s]);
+        if (K) build_fused_program(queue, steps[s], g.M, g.R, K, phases[s], ops[s], blocks, &nblocks);
+        else build_program(queue, steps[s], g.M, g.R, phases[s], ops[s]);
         poff[s] = total;
         total += (phases[s].size() * sizeof(SwPhase) + 255) / 256 * 256;
         ooff[s] = total;
         total += (ops[s].size() * sizeof(SwOp) + 255) / 256 * 256;
     }
+    const size_t boff = total;
+    total += blocks.size() * sizeof(float);
     char* dprog = nullptr;
     if (total) {
         std::vector<char> host(total);
@@ -441,9 +303,11 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             std::memcpy(host.data() + poff[s], phases[s].data(), phases[s].size() * sizeof(SwPhase));
             std::memcpy(host.data() + ooff[s], ops[s].data(), ops[s].size() * sizeof(SwOp));
         }
+        if (!blocks.empty()) std::memcpy(host.data() + boff, blocks.data(), blocks.size() * sizeof(float));
         dprog = static_cast<char*>(st.scratch(GpuVectorState::SLOT_PROGRAM, total));
         SW_HIP_OK(hipMemcpy(dprog, host.data(), total, hipMemcpyHostToDevice));
     }
+    const float* dblocks = reinterpret_cast<const float*>(dprog + boff);
     const double S = double(uint64_t(1) << n) * double(st.elem_bytes()) * double(col_end - col_begin);
     for (size_t s = 0; s < steps.size(); ++s) {
         const PlanStep& step = steps[s];
@@ -454,14 +318,15 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
         const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
         if (st.precision() == PREC_C128)
-            launch_sweep<double2, 12, 4, 4>(st, step, dp, int(phases[s].size()), dop, col_begin, col_end);
+            launch_sweep<double2, 12, 4, 4>(st, step, dp, int(phases[s].size()), dop, dblocks, col_begin, col_end);
         else
-            launch_sweep<float2, 13, 5, 5>(st, step, dp, int(phases[s].size()), dop, col_begin, col_end);
+            launch_sweep<float2, 13, 5, 5>(st, step, dp, int(phases[s].size()), dop, dblocks, col_begin, col_end);
         stats.launches += 1;
         stats.sweeps += 1;
         stats.gates += step.gates.size();
         stats.bytes += 2.0 * S;
     }
+    stats.blocks += uint64_t(nblocks);
 }
 
 }  // namespace q1t

@@ -176,7 +176,7 @@ class GpuVectorState:
         s = _lib.Stats()
         check(lib().q1t_gvs_last_stats(self.h, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates}
+                "gates": s.gates, "blocks": s.blocks}
 
     def set_profiling(self, on=True):
         check(lib().q1t_gvs_set_profiling(self.h, 1 if on else 0))

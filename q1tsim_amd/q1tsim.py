@@ -298,7 +298,7 @@ class Circuit(object):
         s = _lib.Stats()
         _lib.check(self.__sim.q1t_circuit_last_stats(self.__ptr, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates}
+                "gates": s.gates, "blocks": s.blocks}
 
     def histogram_vec(self):
         n = 1 << self.nr_cbits()

@@ -107,6 +107,47 @@ def test_sweep_c64(n):
     assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
 
 
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_fused_c64_blocks(k):
+    """Dense k-qubit blocks composed in FP64, applied with FP32 MFMA."""
+    n = 18
+    a = rand_state(n, 20 + k)
+    ops = mixed_ops(n, 250, 20 + k) + workloads.random_layered(n, 6, k)
+    g = GpuVectorState.from_amplitudes(a, 1, "c64")
+    g.set_plan(k, True)
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    g.flush()
+    st = g.stats()
+    col = g.column(0)
+    ref = oracle(n, ops, a)[:, 0]
+    err = float(np.abs(col - ref).max())
+    nrm = float(np.vdot(col, col).real)
+    infid = 1 - abs(np.vdot(ref, col)) ** 2 / nrm
+    print("fused k=%d: sweeps %d blocks %d max-abs %.2e infidelity %.2e" % (k, st["sweeps"], st["blocks"], err,
+                                                                            infid))
+    assert st["blocks"] > 0
+    assert err <= 1e-5 and infid < 1e-6, (err, infid)
+
+
+def test_fused_layered_config1():
+    n = 18
+    ops = workloads.random_layered(n, 20, 1)
+    g = GpuVectorState(n, 1, "c64")
+    g.set_plan(4, True)
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    col = g.column(0)
+    r = orc.RefState(n, 1)
+    for name, bits, params in ops:
+        r.apply_gate(name, bits, params)
+    ref = r.states[:, 0]
+    nrm = float(np.vdot(col, col).real)
+    infid = 1 - abs(np.vdot(ref, col)) ** 2 / nrm
+    err = float(np.abs(col - ref).max())
+    assert err <= 1e-5 and infid < 1e-6, (err, infid)
+
+
 def test_qft_sweeps():
     n = 20
     g = GpuVectorState(n, 1)
