@@ -86,6 +86,29 @@ int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out);
 /* histogram_vec (circuit.rs:800-815): counts of length 2^nr_cbits */
 int q1t_circuit_histogram_vec(const circuit_t* ptr, size_t* out, size_t len);
 
+/* ---- batches of circuits (no ffi.rs counterpart: the reference runs one
+ * circuit per Circuit::execute, circuit.rs:562-600) ----
+ * Every added circuit is executed as circuit_execute(ptr, nr_shots) would
+ * execute it (its own seed, the same classical state), in lockstep with the
+ * others: the gate runs of all circuits between their measurements are
+ * flushed together, one launch per sweep round.  Circuits must outlive the
+ * batch's execute call; their backend is set to the batch's. */
+typedef struct q1t_batch q1t_batch;
+typedef struct q1t_batch_stats {
+    q1t_stats gates;        /* summed over all gate flushes */
+    uint64_t waves;         /* groups of circuits resident at once */
+    uint64_t rounds;        /* lockstep rounds */
+    uint64_t sync_ops;      /* measurements, conditional gates, resets */
+    double seconds;         /* wall time of the execute call */
+} q1t_batch_stats;
+/* mem_budget: device bytes one wave of states may take (0: 3/4 of free memory) */
+int q1t_batch_new(int precision, int fusion_k, size_t mem_budget, q1t_batch** out);
+void q1t_batch_free(q1t_batch* b);
+int q1t_batch_add(q1t_batch* b, circuit_t* c);
+/* keep_states != 0: leave each end state on the device (circuit reads work) */
+int q1t_batch_execute(q1t_batch* b, size_t nr_shots, int keep_states);
+int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,7 @@ typedef struct {
     uint64_t sweeps;       /* HBM sweeps over the state */
     uint64_t gates;        /* gates applied */
     uint64_t blocks;       /* fused dense blocks applied (per column) */
+    uint64_t phases;       /* in-tile phases (LDS exchanges) over all sweeps */
 } q1t_stats;
 
 const char* q1t_last_error(void);
@@ -127,6 +128,10 @@ int q1t_gvs_flush(q1t_gvs* s);
  * 2..6 composes gate runs into dense k-qubit blocks; tiling = 1 applies
  * runs that fit an LDS tile in one HBM sweep. */
 int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling);
+/* Build-added: tile geometry of the sweeps (M tile bits, R register bits,
+ * L low bits per tile); only the compiled shapes are accepted
+ * (complex128: M=12 with R=3 or 4; complex64: M=12 or 13 with R=4). */
+int q1t_gvs_set_geometry(q1t_gvs* s, int M, int R, int L);
 int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out);
 
 /* Build-added measurement: with profiling on, every kernel launch of a flush

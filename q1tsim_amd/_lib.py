@@ -27,12 +27,18 @@ class Result(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("gate_seconds", ctypes.c_double), ("bytes", ctypes.c_double), ("launches", ctypes.c_uint64),
-                ("sweeps", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("blocks", ctypes.c_uint64)]
+                ("sweeps", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("blocks", ctypes.c_uint64),
+                ("phases", ctypes.c_uint64)]
 
 
 class KernelProfile(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double),
                 ("bytes", ctypes.c_double)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("gates", Stats), ("waves", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("sync_ops", ctypes.c_uint64), ("seconds", ctypes.c_double)]
 
 
 class Rng(ctypes.Structure):
@@ -76,6 +82,11 @@ SIGNATURES = {
     "q1t_circuit_read_amplitudes": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_circuit_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_circuit_histogram_vec": (_i32, [_vp, _ps, _sz]),
+    "q1t_batch_new": (_i32, [_i32, _i32, _sz, ctypes.POINTER(_vp)]),
+    "q1t_batch_free": (None, [_vp]),
+    "q1t_batch_add": (_i32, [_vp, _vp]),
+    "q1t_batch_execute": (_i32, [_vp, _sz, _i32]),
+    "q1t_batch_stats_get": (_i32, [_vp, ctypes.POINTER(BatchStats)]),
     "q1t_gvs_new": (_i32, [_sz, _sz, _i32, ctypes.POINTER(_vp)]),
     "q1t_gvs_free": (None, [_vp]),
     "q1t_gvs_from_qubit_coefs": (_i32, [_vp, _pd, _sz]),
@@ -99,6 +110,7 @@ SIGNATURES = {
     "q1t_gvs_read_amplitudes": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_gvs_flush": (_i32, [_vp]),
     "q1t_gvs_set_plan": (_i32, [_vp, _i32, _i32]),
+    "q1t_gvs_set_geometry": (_i32, [_vp, _i32, _i32, _i32]),
     "q1t_gvs_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_gvs_set_profiling": (_i32, [_vp, _i32]),
     "q1t_gvs_kernel_profile": (_i32, [_vp, _vp, _sz, _ps]),

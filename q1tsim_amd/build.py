@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["gvs.hip", "sweep.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "capi.cpp"]
+SOURCES = ["gvs.hip", "sweep.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp", "capi.cpp"]
 
 
 def _compile(src):

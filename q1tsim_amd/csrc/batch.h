@@ -1,0 +1,50 @@
+// Batches of circuits on one GPU: many independent circuits (uneven qubit
+// counts, branch columns after mid-circuit measurements) executed in
+// lockstep on one stream, the gate runs of all of them flushed together
+// (one launch per sweep round; small states whole in LDS).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "circuit.h"
+#include "sweep.h"
+
+namespace q1t {
+
+struct BatchStats {
+    RunStats gates;            // summed over all flushes of all circuits
+    uint64_t waves = 0;        // groups of circuits resident at once
+    uint64_t rounds = 0;       // lockstep rounds (gate flush + sync ops)
+    uint64_t sync_ops = 0;     // measurements, conditional gates, resets
+    double seconds = 0.0;      // wall time of execute (host clock)
+};
+
+class Batch {
+public:
+    // `mem_budget`: device bytes the resident states of one wave may take
+    // (upper bound over their branch columns); 0 = 3/4 of free memory.
+    Batch(Precision prec, int fusion_k, size_t mem_budget);
+
+    // The circuit must stay alive until execute returns; its backend
+    // precision and fusion are set to the batch's.
+    void add(Circuit* c);
+    size_t size() const { return circuits_.size(); }
+    // Run every circuit as Circuit::execute(nr_shots) would (same seeds, same
+    // classical results); `keep_states` keeps each end state on the device.
+    void execute(size_t nr_shots, bool keep_states);
+    const BatchStats& stats() const { return stats_; }
+
+private:
+    Precision prec_;
+    int fusion_k_;
+    size_t mem_budget_;
+    std::vector<Circuit*> circuits_;
+    std::shared_ptr<SharedStream> stream_;
+    DeviceBuffer prog_;
+    BatchStats stats_;
+
+    size_t resident_bytes(const Circuit& c, size_t nr_shots) const;
+};
+
+}  // namespace q1t

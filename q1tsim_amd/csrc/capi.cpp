@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "../../include/q1tsim.h"
+#include "batch.h"
 #include "circuit.h"
 
 using namespace q1t;
@@ -22,6 +23,11 @@ struct q1t_gvs {
 
 struct q1t_xoshiro {
     Xoshiro256 x;
+};
+
+struct q1t_batch {
+    Batch b;
+    q1t_batch(Precision p, int k, size_t budget) : b(p, k, budget) {}
 };
 
 static thread_local std::string g_last_error;
@@ -323,7 +329,7 @@ extern "C" int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out)
     NEED(ptr);
     NEED(out);
     const RunStats& r = ptr->c.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
     return 0;
 }
 
@@ -336,6 +342,48 @@ extern "C" int q1t_circuit_histogram_vec(const circuit_t* ptr, size_t* out, size
         if (len < h.size()) throw err_not_enough_space(len, h.size());
         std::copy(h.begin(), h.end(), out);
     });
+}
+
+// ------------------------------------------------- batches
+
+extern "C" int q1t_batch_new(int precision, int fusion_k, size_t mem_budget, q1t_batch** out)
+{
+    NEED(out);
+    *out = nullptr;
+    return guard([&] {
+        if (precision != PREC_C128 && precision != PREC_C64) throw Error(ERR_INVALID_ARGUMENT, "Unknown precision");
+        if (fusion_k < 0 || fusion_k > 6) throw Error(ERR_INVALID_ARGUMENT, "fusion_k must be 0..6");
+        *out = new q1t_batch(Precision(precision), fusion_k, mem_budget);
+    });
+}
+
+extern "C" void q1t_batch_free(q1t_batch* b) { delete b; }
+
+extern "C" int q1t_batch_add(q1t_batch* b, circuit_t* c)
+{
+    NEED(b);
+    NEED(c);
+    return guard([&] { b->b.add(&c->c); });
+}
+
+extern "C" int q1t_batch_execute(q1t_batch* b, size_t nr_shots, int keep_states)
+{
+    NEED(b);
+    return guard([&] { b->b.execute(nr_shots, keep_states != 0); });
+}
+
+extern "C" int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out)
+{
+    NEED(b);
+    NEED(out);
+    const BatchStats& s = b->b.stats();
+    const RunStats& r = s.gates;
+    out->gates = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
+    out->waves = s.waves;
+    out->rounds = s.rounds;
+    out->sync_ops = s.sync_ops;
+    out->seconds = s.seconds;
+    return 0;
 }
 
 // ------------------------------------------------- state-level ABI
@@ -569,12 +617,18 @@ extern "C" int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling)
     });
 }
 
+extern "C" int q1t_gvs_set_geometry(q1t_gvs* s, int M, int R, int L)
+{
+    NEED(s);
+    return guard([&] { s->s.set_geometry(TileGeom{M, R, L}); });
+}
+
 extern "C" int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out)
 {
     NEED(s);
     NEED(out);
     const RunStats& r = s->s.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
     return 0;
 }
 

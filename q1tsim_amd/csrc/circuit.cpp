@@ -110,15 +110,21 @@ void Circuit::set_initial_random(uint64_t seed)
 // execute / execute_with (circuit.rs:562-600): fresh state and zeroed c_state
 void Circuit::execute(size_t nr_shots)
 {
+    start(nr_shots, nullptr);
+    do_execute();
+}
+
+void Circuit::start(size_t nr_shots, std::shared_ptr<SharedStream> stream)
+{
     state_.reset();
-    state_ = std::make_unique<GpuVectorState>(nr_qbits_, nr_shots, prec_);
+    state_ = std::make_unique<GpuVectorState>(nr_qbits_, nr_shots, prec_, std::move(stream));
     state_->set_fusion(fusion_k_);
     state_->set_tiling(tiling_);
     if (init_mode_ == 1) state_->set_amplitudes(init_amps_.data());
     if (init_mode_ == 2) state_->set_random_normal(init_seed_);
     c_state_.assign(nr_shots, 0);
     has_cstate_ = true;
-    do_execute();
+    cursor_ = 0;
 }
 
 // reexecute (circuit.rs:618-641): continue from the previous end state
@@ -142,80 +148,113 @@ void Circuit::apply_named_all(GpuVectorState& st, const char* name)
     st.apply_unary_gate_all(m.data(), g->desc);
 }
 
-// do_execute_with (circuit.rs:643-762)
+// do_execute_with (circuit.rs:643-762): the op loop, gates queued on the
+// state until an op needs their result
 void Circuit::do_execute()
+{
+    cursor_ = 0;
+    while (queue_gates()) run_next_op();
+    finish();
+}
+
+bool Circuit::queue_gates()
+{
+    GpuVectorState& st = *state_;
+    std::vector<double> pv;
+    for (; cursor_ < ops_.size(); ++cursor_) {
+        const CircuitOp& op = ops_[cursor_];
+        if (op.type == OpType::Barrier) continue;
+        if (op.type != OpType::Gate) return true;
+        pv.clear();
+        for (const Param& p : op.params) pv.push_back(p.get());
+        std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+        st.apply_gate(m.data(), gate_nbits(*op.gate), op.bits.data(), op.bits.size(), op.gate->desc);
+    }
+    return false;
+}
+
+void Circuit::run_next_op()
+{
+    if (cursor_ >= ops_.size()) return;
+    run_op(ops_[cursor_]);
+    ++cursor_;
+}
+
+void Circuit::finish()
+{
+    state_->flush();
+    stats_ = state_->last_stats();
+}
+
+void Circuit::run_op(const CircuitOp& op)
 {
     GpuVectorState& st = *state_;
     RngSource rng = rng_.source();
     std::vector<double> pv;
-    for (const CircuitOp& op : ops_) {
-        switch (op.type) {
-        case OpType::Gate: {
-            pv.clear();
-            for (const Param& p : op.params) pv.push_back(p.get());
-            std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
-            st.apply_gate(m.data(), gate_nbits(*op.gate), op.bits.data(), op.bits.size(), op.gate->desc);
-            break;
-        }
-        case OpType::CondGate: {
-            // control word: bit i of the word is c_state bit control[i] (circuit.rs:655-665)
-            std::vector<uint8_t> apply(c_state_.size());
-            for (size_t r = 0; r < c_state_.size(); ++r) {
-                uint64_t w = 0;
-                for (size_t i = 0; i < op.control.size(); ++i) w |= ((c_state_[r] >> op.control[i]) & 1) << i;
-                apply[r] = w == op.target;
-            }
-            pv.clear();
-            for (const Param& p : op.params) pv.push_back(p.get());
-            std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
-            st.apply_conditional_gate(apply.data(), apply.size(), m.data(), gate_nbits(*op.gate), op.bits.data(),
-                                      op.bits.size(), op.gate->desc);
-            break;
-        }
-        case OpType::Measure:
-        case OpType::Peek: {
-            const bool collapse = op.type == OpType::Measure;
-            if (op.basis == 'X') apply_named(st, "h", op.qbit);
-            if (op.basis == 'Y') {
-                apply_named(st, "sdg", op.qbit);
-                apply_named(st, "h", op.qbit);
-            }
-            st.measure_into(op.qbit, op.cbit, c_state_.data(), c_state_.size(), rng, collapse);
-            if (op.basis == 'X') apply_named(st, "h", op.qbit);
-            if (op.basis == 'Y') {
-                apply_named(st, "h", op.qbit);
-                apply_named(st, "s", op.qbit);
-            }
-            break;
-        }
-        case OpType::MeasureAll:
-        case OpType::PeekAll: {
-            const bool collapse = op.type == OpType::MeasureAll;
-            if (op.basis == 'X') apply_named_all(st, "h");
-            if (op.basis == 'Y') {
-                apply_named_all(st, "sdg");
-                apply_named_all(st, "h");
-            }
-            st.measure_all_into(op.cbits.data(), op.cbits.size(), c_state_.data(), c_state_.size(), rng, collapse);
-            if (op.basis == 'X') apply_named_all(st, "h");
-            if (op.basis == 'Y') {
-                apply_named_all(st, "h");
-                apply_named_all(st, "s");
-            }
-            break;
-        }
-        case OpType::Reset:
-            st.reset(op.qbit, rng);
-            break;
-        case OpType::ResetAll:
-            st.reset_all();
-            break;
-        case OpType::Barrier:
-            break;
-        }
+    switch (op.type) {
+    case OpType::Gate: {
+        pv.clear();
+        for (const Param& p : op.params) pv.push_back(p.get());
+        std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+        st.apply_gate(m.data(), gate_nbits(*op.gate), op.bits.data(), op.bits.size(), op.gate->desc);
+        break;
     }
-    st.flush();
-    stats_ = st.last_stats();
+    case OpType::CondGate: {
+        // control word: bit i of the word is c_state bit control[i] (circuit.rs:655-665)
+        std::vector<uint8_t> apply(c_state_.size());
+        for (size_t r = 0; r < c_state_.size(); ++r) {
+            uint64_t w = 0;
+            for (size_t i = 0; i < op.control.size(); ++i) w |= ((c_state_[r] >> op.control[i]) & 1) << i;
+            apply[r] = w == op.target;
+        }
+        pv.clear();
+        for (const Param& p : op.params) pv.push_back(p.get());
+        std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+        st.apply_conditional_gate(apply.data(), apply.size(), m.data(), gate_nbits(*op.gate), op.bits.data(),
+                                  op.bits.size(), op.gate->desc);
+        break;
+    }
+    case OpType::Measure:
+    case OpType::Peek: {
+        const bool collapse = op.type == OpType::Measure;
+        if (op.basis == 'X') apply_named(st, "h", op.qbit);
+        if (op.basis == 'Y') {
+            apply_named(st, "sdg", op.qbit);
+            apply_named(st, "h", op.qbit);
+        }
+        st.measure_into(op.qbit, op.cbit, c_state_.data(), c_state_.size(), rng, collapse);
+        if (op.basis == 'X') apply_named(st, "h", op.qbit);
+        if (op.basis == 'Y') {
+            apply_named(st, "h", op.qbit);
+            apply_named(st, "s", op.qbit);
+        }
+        break;
+    }
+    case OpType::MeasureAll:
+    case OpType::PeekAll: {
+        const bool collapse = op.type == OpType::MeasureAll;
+        if (op.basis == 'X') apply_named_all(st, "h");
+        if (op.basis == 'Y') {
+            apply_named_all(st, "sdg");
+            apply_named_all(st, "h");
+        }
+        st.measure_all_into(op.cbits.data(), op.cbits.size(), c_state_.data(), c_state_.size(), rng, collapse);
+        if (op.basis == 'X') apply_named_all(st, "h");
+        if (op.basis == 'Y') {
+            apply_named_all(st, "h");
+            apply_named_all(st, "s");
+        }
+        break;
+    }
+    case OpType::Reset:
+        st.reset(op.qbit, rng);
+        break;
+    case OpType::ResetAll:
+        st.reset_all();
+        break;
+    case OpType::Barrier:
+        break;
+    }
 }
 
 // histogram_string (circuit.rs:823-841): binary keys of width nr_cbits, cbit 0 last

@@ -52,6 +52,21 @@ public:
     void execute(size_t nr_shots);
     void reexecute();
 
+    // Resumable execution (batches): `start` makes the fresh state that
+    // `execute` makes, on `stream`; `queue_gates` queues the gate ops from the
+    // cursor on, up to the next op that needs their result; `run_next_op`
+    // applies that op; `finish` flushes and keeps the stats.  `release_state`
+    // drops the state (the classical state stays).
+    void start(size_t nr_shots, std::shared_ptr<SharedStream> stream);
+    bool queue_gates();
+    void run_next_op();
+    void finish();
+    void release_state() { state_.reset(); }
+    size_t nr_ops() const { return ops_.size(); }
+    const std::vector<CircuitOp>& ops() const { return ops_; }
+    Precision precision() const { return prec_; }
+    int fusion() const { return fusion_k_; }
+
     std::vector<std::pair<std::string, size_t>> histogram_string() const;
     std::vector<size_t> histogram_vec() const;
 
@@ -78,8 +93,10 @@ private:
     std::vector<cdouble> init_amps_;
     uint64_t init_seed_ = 0;
     RunStats stats_;
+    size_t cursor_ = 0;
 
     void do_execute();
+    void run_op(const CircuitOp& op);
     void apply_named(GpuVectorState& st, const char* name, size_t q);
     void apply_named_all(GpuVectorState& st, const char* name);
 };

@@ -21,6 +21,14 @@ struct RunStats {
     uint64_t sweeps = 0;          // full-state passes
     uint64_t gates = 0;           // gates applied
     uint64_t blocks = 0;          // fused dense blocks applied per column
+    uint64_t phases = 0;          // in-tile phases over all sweeps
+};
+
+// Tile geometry of the sweeps: M local bits per tile, R of them in registers
+// per phase (2^(M-R) threads), L lowest index bits always in the tile (runs
+// of 2^L contiguous amplitudes per HBM access).
+struct TileGeom {
+    int M, R, L;
 };
 
 // Kernel time measured with HIP events on the state's stream.
@@ -42,9 +50,21 @@ struct PendingGate {
     std::vector<int> qbits;       // index-bit positions of the listed qubits
 };
 
+// A HIP stream (default priority, non-blocking) shared by the states of a
+// batch; destroyed with its last user.
+struct SharedStream {
+    hipStream_t s = nullptr;
+    SharedStream();
+    ~SharedStream();
+    SharedStream(const SharedStream&) = delete;
+    SharedStream& operator=(const SharedStream&) = delete;
+};
+
 class GpuVectorState {
 public:
-    GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec);
+    // `shared`: run on a stream shared with other states (batches) instead
+    // of a stream of its own.
+    GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared = nullptr);
     ~GpuVectorState();
     GpuVectorState(const GpuVectorState&) = delete;
     GpuVectorState& operator=(const GpuVectorState&) = delete;
@@ -83,9 +103,17 @@ public:
     void read_amplitudes(size_t col, cdouble* out);
     void flush();
     void synchronize();
+    // batched flushes (many states, one launch per sweep round): the queue is
+    // taken over by the batch runner, which applies it and reports stats here
+    std::vector<PendingGate>& pending() { return queue_; }
+    void prepare_batched_flush();
+    void finish_batched_flush(const RunStats& s);
     const RunStats& last_stats() const { return stats_; }
     void set_fusion(int max_block_bits) { fusion_k_ = max_block_bits; }
+    int fusion() const { return fusion_k_; }
     void set_tiling(bool on) { tiling_ = on; }
+    void set_geometry(const TileGeom& g);
+    const TileGeom& geometry() const { return geom_; }
     hipStream_t stream() const { return stream_; }
     void* device_amplitudes() { return amp_; }
     size_t elem_bytes() const { return prec_ == PREC_C128 ? 16 : 8; }
@@ -114,10 +142,12 @@ private:
     void* amp_ = nullptr;                 // column-major: column c at amp_ + c * 2^n elements
     size_t amp_cols_ = 0;                 // allocated columns
     hipStream_t stream_ = nullptr;
+    std::shared_ptr<SharedStream> shared_;   // set: stream_ belongs to it
     std::vector<PendingGate> queue_;
     RunStats stats_;
     int fusion_k_ = 0;                    // 0: plain per-gate arithmetic
-    bool tiling_ = false;
+    bool tiling_ = true;
+    TileGeom geom_{12, 4, 4};
 
     // scratch
     double* d_partial_ = nullptr;

@@ -417,15 +417,52 @@ template <typename V> static V to_v(const cdouble& c)
 
 // ------------------------------------------------------------ the state
 
-GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec)
+SharedStream::SharedStream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+
+SharedStream::~SharedStream()
+{
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+}
+
+GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared)
     : nr_bits_(nr_bits), nr_shots_(nr_shots), prec_(prec)
 {
     if (nr_bits > 40) throw Error(ERR_INVALID_ARGUMENT, "Too many qubits for a dense state");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw Error(ERR_HIP, "No HIP device available for the GPU statevector");
-    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (shared) {
+        shared_ = std::move(shared);
+        stream_ = shared_->s;
+    } else {
+        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    }
+    geom_ = tile_geometry(prec);
     set_zero_state();
+}
+
+void GpuVectorState::prepare_batched_flush()
+{
+    materialize();
+    stats_ = RunStats{};
+}
+
+void GpuVectorState::finish_batched_flush(const RunStats& s)
+{
+    stats_ = s;
+    queue_.clear();
+}
+
+void GpuVectorState::set_geometry(const TileGeom& g)
+{
+    if (!tile_shape_supported(prec_, g.M, g.R) || g.L < 1 || g.L > g.M)
+        throw Error(ERR_INVALID_ARGUMENT, "Unsupported tile geometry M=" + std::to_string(g.M) + " R=" +
+                                              std::to_string(g.R) + " L=" + std::to_string(g.L));
+    flush();
+    geom_ = g;
 }
 
 GpuVectorState::~GpuVectorState()
@@ -441,7 +478,7 @@ GpuVectorState::~GpuVectorState()
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (stream_ && !shared_) (void)hipStreamDestroy(stream_);
 }
 
 void GpuVectorState::check_qbit(size_t q) const

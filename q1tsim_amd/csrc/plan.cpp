@@ -16,10 +16,16 @@ namespace q1t {
 
 TileGeom tile_geometry(Precision p)
 {
-    // 64 KiB of LDS per tile either way (two workgroups per CU); 16 amplitudes
-    // per thread for complex128, 32 for complex64; 256 threads either way.
-    if (p == PREC_C128) return TileGeom{12, 4, 4};
-    return TileGeom{13, 5, 5};
+    // 2^12-amplitude tiles, 16 amplitudes per thread, 256 threads; runs of
+    // 16 contiguous amplitudes.  Other compiled shapes: tile_shape_supported.
+    (void)p;
+    return TileGeom{12, 4, 4};
+}
+
+bool tile_shape_supported(Precision p, int M, int R)
+{
+    if (p == PREC_C128) return M == 12 && (R == 4 || R == 3);
+    return (M == 12 && R == 4) || (M == 13 && R == 4);
 }
 
 static std::vector<int> mask_list(uint64_t m)
@@ -177,6 +183,17 @@ SwPhase make_phase(uint32_t kind, uint32_t regs, int M, uint32_t op_begin)
     return ph;
 }
 
+// The cheapest exact body for a 2x2 matrix (row-major m00, m01, m10, m11).
+uint32_t dense_form(const cdouble* m)
+{
+    const cdouble z(0, 0), o(1, 0);
+    if (m[0] == z && m[1] == o && m[2] == o && m[3] == z) return SWF_SWAP;
+    if (m[0] == z && m[1] == cdouble(0, -1) && m[2] == cdouble(0, 1) && m[3] == z) return SWF_Y;
+    if (m[0].imag() == 0 && m[1].imag() == 0 && m[2].imag() == 0 && m[3].imag() == 0) return SWF_REAL;
+    if (m[0].imag() == 0 && m[0] == m[3] && m[1].real() == 0 && m[1] == m[2]) return SWF_RX;
+    return SWF_GENERAL;
+}
+
 SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 {
     SwOp o;
@@ -194,10 +211,13 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
             o.m[2 * k] = op.m[k].real();
             o.m[2 * k + 1] = op.m[k].imag();
         }
+        o.aux = dense_form(op.m.data());
     } else if (op.kind == OpKind::PhaseOnes) {
         o.type = SW_PHASE;
         o.m[0] = op.m[0].real();
         o.m[1] = op.m[0].imag();
+        const cdouble p = op.m[0];
+        o.aux = p == cdouble(0, 1) ? SWP_I : p == cdouble(0, -1) ? SWP_MINUS_I : p == cdouble(-1, 0) ? SWP_NEG : SWP_GENERAL;
     } else {  // Diag on one target
         o.type = SW_DIAG1;
         const int t = op.targets[0];

@@ -11,6 +11,11 @@ namespace q1t {
 // workgroup; uniform).
 enum SwOpType : int32_t { SW_DENSE1 = 0, SW_PHASE = 1, SW_DIAG1 = 2, SW_BLOCK = 3 };
 
+// Body of a SW_DENSE1 op, read off its 2x2 matrix (SwOp::aux).
+enum SwDenseForm : uint32_t { SWF_GENERAL = 0, SWF_REAL = 1, SWF_SWAP = 2, SWF_RX = 3, SWF_Y = 4 };
+// Body of a SW_PHASE op (SwOp::aux): general, times i, times -i, times -1.
+enum SwPhaseForm : uint32_t { SWP_GENERAL = 0, SWP_I = 1, SWP_MINUS_I = 2, SWP_NEG = 3 };
+
 struct SwOp {
     int32_t type;
     int32_t reg;       // SW_DENSE1: index of the target among the phase's register bits
@@ -37,12 +42,10 @@ struct PlanStep {
     std::vector<size_t> gates;      // indices into the queue, in application order
 };
 
-// Tile geometry per precision: M local bits, R register bits, L low bits kept
-// in every tile for coalesced 2^L-element runs.
-struct TileGeom {
-    int M, R, L;
-};
+// Default tile geometry per precision.
 TileGeom tile_geometry(Precision p);
+// Is (M, R) one of the compiled tile shapes for this precision?
+bool tile_shape_supported(Precision p, int M, int R);
 
 // Is a gate applicable inside a tile (targets that must be local, if any)?
 bool sweepable(const GateOp& op, std::vector<int>* local_targets);
@@ -62,6 +65,22 @@ void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int 
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
                          int* nblocks);
+
+// Device buffer that grows on demand (synchronising `stream` first).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes, hipStream_t stream);
+    ~DeviceBuffer();
+};
+
+// Flush the pending gates of many states together (all of one precision,
+// tile geometry and fusion setting, sharing `stream`): states at least
+// M+2 qubits wide are planned into sweeps and the r-th sweep of every state
+// runs in one launch over all their tiles; smaller states run whole in LDS,
+// one workgroup per column, all in one launch.
+void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
+                       RunStats& agg);
 
 // Apply the pending gates to columns [col_begin, col_end) of the state with
 // the planned schedule: runs of gates whose qubits fit one tile are applied

@@ -48,7 +48,11 @@ template <int SW, int M> __device__ __forceinline__ int swz(int l)
     return l ^ (f & ((1 << SW) - 1));
 }
 
-template <typename V, int NR, int J>
+// One-qubit update of the register pairs (k, k | 2^J) whose control bits are
+// set.  FORM picks the body from the matrix structure, as the reference's
+// per-gate apply_mat_slice bodies do: general complex, real (H, RY), swap
+// (X, CX), [[a, ib], [ib, a]] (RX) and [[0, -i], [i, 0]] (Y).
+template <typename V, int NR, int J, int FORM>
 __device__ __forceinline__ void dense_pairs(V (&v)[NR], const uint32_t (&lidx)[NR], uint32_t cl, V m0, V m1, V m2, V m3)
 {
 #pragma unroll
@@ -57,9 +61,39 @@ __device__ __forceinline__ void dense_pairs(V (&v)[NR], const uint32_t (&lidx)[N
         const int k1 = k | (1 << J);
         if ((lidx[k] & cl) == cl) {
             const V x = v[k], y = v[k1];
-            v[k] = cadd(cmul(m0, x), cmul(m1, y));
-            v[k1] = cadd(cmul(m2, x), cmul(m3, y));
+            if constexpr (FORM == SWF_GENERAL) {
+                v[k] = cadd(cmul(m0, x), cmul(m1, y));
+                v[k1] = cadd(cmul(m2, x), cmul(m3, y));
+            } else if constexpr (FORM == SWF_REAL) {
+                v[k] = mkc<V>(m0.x * x.x + m1.x * y.x, m0.x * x.y + m1.x * y.y);
+                v[k1] = mkc<V>(m2.x * x.x + m3.x * y.x, m2.x * x.y + m3.x * y.y);
+            } else if constexpr (FORM == SWF_SWAP) {
+                v[k] = y;
+                v[k1] = x;
+            } else if constexpr (FORM == SWF_RX) {
+                const auto a = m0.x, b = m1.y;
+                v[k] = mkc<V>(a * x.x - b * y.y, a * x.y + b * y.x);
+                v[k1] = mkc<V>(a * y.x - b * x.y, a * y.y + b * x.x);
+            } else {  // SWF_Y
+                v[k] = mkc<V>(y.y, -y.x);
+                v[k1] = mkc<V>(-x.y, x.x);
+            }
         }
+    }
+}
+
+template <typename V, int NR, int R, int FORM>
+__device__ __forceinline__ void dense_reg(int reg, V (&v)[NR], const uint32_t (&lidx)[NR], uint32_t cl, V m0, V m1,
+                                          V m2, V m3)
+{
+    switch (reg) {
+    case 0: dense_pairs<V, NR, 0, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
+    case 1: dense_pairs<V, NR, 1, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
+    case 2: dense_pairs<V, NR, 2, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
+    case 3: dense_pairs<V, NR, 3, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
+    default:
+        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM>(v, lidx, cl, m0, m1, m2, m3);
+        break;
     }
 }
 
@@ -127,16 +161,13 @@ __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict_
     }
 }
 
+// One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
 template <typename V, int M, int R, int SW>
-__global__ void __launch_bounds__(1 << (M - R))
-__attribute__((amdgpu_waves_per_eu(2)))
-k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
+__device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& sa, uint64_t tile, V* lds)
 {
     constexpr int NT = 1 << (M - R), NR = 1 << R;
     using Rl = typename RealOf<V>::type;
-    __shared__ V lds[1 << M];
     const int tid = threadIdx.x;
-    V* a = amp + uint64_t(blockIdx.y) * col_stride;
 
     // global offset of this thread's part of the natural local order
     uint64_t g_tid = 0;
@@ -153,7 +184,7 @@ k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
         g_k[k] = gk;
     }
 
-    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) {
+    {
         const uint64_t base = deposit(tile, sa.tset);
 #pragma unroll
         for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | g_tid | g_k[k]];
@@ -197,20 +228,38 @@ k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
                 if (op.type == SW_DENSE1) {
                     const V m0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), m1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
                     const V m2 = mkc<V>(Rl(op.m[4]), Rl(op.m[5])), m3 = mkc<V>(Rl(op.m[6]), Rl(op.m[7]));
-                    switch (op.reg) {
-                    case 0: dense_pairs<V, NR, 0>(v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case 1: dense_pairs<V, NR, 1>(v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case 2: dense_pairs<V, NR, 2>(v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case 3: dense_pairs<V, NR, 3>(v, lidx, op.cl, m0, m1, m2, m3); break;
-                    default:
-                        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0)>(v, lidx, op.cl, m0, m1, m2, m3);
-                        break;
+                    switch (op.aux) {
+                    case SWF_REAL: dense_reg<V, NR, R, SWF_REAL>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
+                    case SWF_SWAP: dense_reg<V, NR, R, SWF_SWAP>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
+                    case SWF_RX: dense_reg<V, NR, R, SWF_RX>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
+                    case SWF_Y: dense_reg<V, NR, R, SWF_Y>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
+                    default: dense_reg<V, NR, R, SWF_GENERAL>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
                     }
                 } else if (op.type == SW_PHASE) {
                     const V ph1 = mkc<V>(Rl(op.m[0]), Rl(op.m[1]));
+                    const uint32_t cl = op.cl;
+                    switch (op.aux) {
+                    case SWP_I:
 #pragma unroll
-                    for (int k = 0; k < NR; ++k)
-                        if ((lidx[k] & op.cl) == op.cl) v[k] = cmul(ph1, v[k]);
+                        for (int k = 0; k < NR; ++k)
+                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(-v[k].y, v[k].x);
+                        break;
+                    case SWP_MINUS_I:
+#pragma unroll
+                        for (int k = 0; k < NR; ++k)
+                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(v[k].y, -v[k].x);
+                        break;
+                    case SWP_NEG:
+#pragma unroll
+                        for (int k = 0; k < NR; ++k)
+                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(-v[k].x, -v[k].y);
+                        break;
+                    default:
+#pragma unroll
+                        for (int k = 0; k < NR; ++k)
+                            if ((lidx[k] & cl) == cl) v[k] = cmul(ph1, v[k]);
+                        break;
+                    }
                 } else {  // SW_DIAG1
                     const V d0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), d1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
                     if (op.dlocal >= 0) {
@@ -233,6 +282,42 @@ k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 #pragma unroll
         for (int k = 0; k < NR; ++k) a[base | g_tid | g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
         __syncthreads();
+    }
+}
+
+template <typename V, int M, int R, int SW>
+__global__ void __launch_bounds__(1 << (M - R))
+__attribute__((amdgpu_waves_per_eu(2)))
+k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
+{
+    __shared__ V lds[1 << M];
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) sweep_tile<V, M, R, SW>(a, sa, tile, lds);
+}
+
+// Sweeps of many states in one launch: item i covers flattened tiles
+// [first_tile[i], first_tile[i+1]) of its own column with its own program.
+struct SweepItem {
+    void* amp;
+    uint64_t first_tile;
+    SweepArgs sa;
+};
+
+template <typename V, int M, int R, int SW>
+__global__ void __launch_bounds__(1 << (M - R))
+__attribute__((amdgpu_waves_per_eu(2)))
+k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_tiles)
+{
+    __shared__ V lds[1 << M];
+    for (uint64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = nitems - 1;   // last item with first_tile <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].first_tile <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const SweepItem& it = items[lo];
+        sweep_tile<V, M, R, SW>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
     }
 }
 
@@ -268,7 +353,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
                  int fusion_k, bool tiling, RunStats& stats)
 {
     const int n = int(st.nr_bits());
-    const TileGeom g = tile_geometry(st.precision());
+    const TileGeom g = st.geometry();
     if (!tiling) {
         for (const PendingGate& pg : queue) st.apply_op(pg.op, col_begin, col_end);
         return;
@@ -317,16 +402,419 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         }
         const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
         const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
-        if (st.precision() == PREC_C128)
-            launch_sweep<double2, 12, 4, 4>(st, step, dp, int(phases[s].size()), dop, dblocks, col_begin, col_end);
-        else
-            launch_sweep<float2, 13, 5, 5>(st, step, dp, int(phases[s].size()), dop, dblocks, col_begin, col_end);
+        const int np = int(phases[s].size());
+        // the compiled tile shapes (M local bits, R register bits): see tile_shapes()
+        if (st.precision() == PREC_C128) {
+            if (g.M == 12 && g.R == 3) launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+        } else {
+            if (g.M == 13) launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+        }
         stats.launches += 1;
         stats.sweeps += 1;
         stats.gates += step.gates.size();
         stats.bytes += 2.0 * S;
+        stats.phases += phases[s].size();
     }
     stats.blocks += uint64_t(nblocks);
+}
+
+// ------------------------------------------------------------- batches
+
+void* DeviceBuffer::get(size_t bytes, hipStream_t stream)
+{
+    if (bytes > cap) {
+        SW_HIP_OK(hipStreamSynchronize(stream));
+        if (ptr) SW_HIP_OK(hipFree(ptr));
+        ptr = nullptr;
+        cap = 0;
+        const size_t c = std::max<size_t>(bytes, size_t(1) << 20);
+        SW_HIP_OK(hipMalloc(&ptr, c));
+        cap = c;
+    }
+    return ptr;
+}
+
+DeviceBuffer::~DeviceBuffer()
+{
+    if (ptr) (void)hipFree(ptr);
+}
+
+// A gate of a small state, applied in LDS.
+struct SmallOp {
+    int32_t kind;      // 0: dense on nt targets, 1: diagonal on nt targets, 2: phase on cmask, 3: swap t0/t1
+    int32_t nt;
+    int32_t t[3];      // target index bits, t[0] = local MSB
+    int32_t npos;      // deposit positions (targets and controls, ascending)
+    int32_t pos[14];
+    uint64_t cmask;
+    uint32_t moff;     // offset of the matrix / diagonal / phase in the value table
+    uint32_t pad;
+};
+
+struct SmallItem {
+    void* amp;
+    int32_t n;
+    int32_t op_begin, op_end;
+    int32_t pad;
+};
+
+__device__ __forceinline__ uint32_t deposit_small(uint32_t j, const SmallOp& op)
+{
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+        if (k < op.npos) {
+            const int s = op.pos[k];
+            j = ((j >> s) << (s + 1)) | (j & ((1u << s) - 1));
+        }
+    }
+    return j;
+}
+
+// Whole states in LDS, one workgroup per (state, column) item; gates one
+// after the other with a barrier in between.
+template <typename V>
+__global__ void __launch_bounds__(256) k_small_batch(const SmallItem* __restrict__ items, int nitems,
+                                                     const SmallOp* __restrict__ ops, const double2* __restrict__ vals)
+{
+    extern __shared__ unsigned char smem_small[];
+    V* s = reinterpret_cast<V*>(smem_small);
+    using Rl = typename RealOf<V>::type;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const SmallItem& item = items[it];
+        V* a = static_cast<V*>(item.amp);
+        const uint32_t N = 1u << item.n;
+        for (uint32_t i = tid; i < N; i += bd) s[i] = a[i];
+        __syncthreads();
+        for (int o = item.op_begin; o < item.op_end; ++o) {
+            const SmallOp& op = ops[o];
+            const double2* mv = vals + op.moff;
+            if (op.kind == 0) {
+                const int nt = op.nt, D = 1 << nt;
+                const uint32_t nwork = N >> op.npos;
+                for (uint32_t w = tid; w < nwork; w += bd) {
+                    const uint32_t base = deposit_small(w, op) | uint32_t(op.cmask);
+                    V x[8], y[8];
+                    uint32_t ix[8];
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) {
+                        if (l < D) {
+                            uint32_t i = base;
+#pragma unroll
+                            for (int q = 0; q < 3; ++q)
+                                if (q < nt && ((l >> (nt - 1 - q)) & 1)) i |= 1u << op.t[q];
+                            ix[l] = i;
+                            x[l] = s[i];
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        if (r < D) {
+                            V acc = mkc<V>(0, 0);
+#pragma unroll
+                            for (int c = 0; c < 8; ++c)
+                                if (c < D) {
+                                    const double2 m = mv[r * D + c];
+                                    acc = cmac(acc, mkc<V>(Rl(m.x), Rl(m.y)), x[c]);
+                                }
+                            y[r] = acc;
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        if (r < D) s[ix[r]] = y[r];
+                }
+            } else if (op.kind == 1) {
+                const uint32_t nwork = N >> op.npos;
+                for (uint32_t w = tid; w < nwork; w += bd) {
+                    const uint32_t i = deposit_small(w, op) | uint32_t(op.cmask);
+                    int l = 0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+                        if (q < op.nt) l = (l << 1) | int((i >> op.t[q]) & 1);
+                    const double2 d = mv[l];
+                    s[i] = cmul(mkc<V>(Rl(d.x), Rl(d.y)), s[i]);
+                }
+            } else if (op.kind == 2) {
+                const uint32_t nwork = N >> op.npos;
+                const double2 d = mv[0];
+                const V ph = mkc<V>(Rl(d.x), Rl(d.y));
+                for (uint32_t w = tid; w < nwork; w += bd) {
+                    const uint32_t i = deposit_small(w, op) | uint32_t(op.cmask);
+                    s[i] = cmul(ph, s[i]);
+                }
+            } else {
+                const uint32_t nwork = N >> op.npos;
+                const uint32_t b0 = 1u << op.t[0], b1 = 1u << op.t[1];
+                for (uint32_t w = tid; w < nwork; w += bd) {
+                    const uint32_t i = deposit_small(w, op) | uint32_t(op.cmask);
+                    const V x = s[i | b1];
+                    s[i | b1] = s[i | b0];
+                    s[i | b0] = x;
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t i = tid; i < N; i += bd) a[i] = s[i];
+        __syncthreads();
+    }
+}
+
+static SmallOp make_small_op(const GateOp& g, std::vector<double2>& vals)
+{
+    SmallOp o;
+    std::memset(&o, 0, sizeof(o));
+    std::vector<int> pos;
+    for (int b = 0; b < 64; ++b)
+        if ((g.ctrl_mask >> b) & 1) pos.push_back(b);
+    o.cmask = g.ctrl_mask;
+    o.moff = uint32_t(vals.size());
+    auto push = [&](const cdouble& c) { vals.push_back(double2{c.real(), c.imag()}); };
+    switch (g.kind) {
+    case OpKind::Dense:
+        o.kind = 0;
+        o.nt = int(g.targets.size());
+        for (int q = 0; q < o.nt; ++q) {
+            o.t[q] = g.targets[q];
+            pos.push_back(g.targets[q]);
+        }
+        for (const cdouble& c : g.m) push(c);
+        break;
+    case OpKind::Diag:
+        o.kind = 1;
+        o.nt = int(g.targets.size());
+        for (int q = 0; q < o.nt; ++q) o.t[q] = g.targets[q];
+        for (const cdouble& c : g.m) push(c);
+        break;
+    case OpKind::PhaseOnes:
+        o.kind = 2;
+        push(g.m[0]);
+        break;
+    case OpKind::Swap2:
+        o.kind = 3;
+        o.nt = 2;
+        o.t[0] = g.targets[0];
+        o.t[1] = g.targets[1];
+        pos.push_back(g.targets[0]);
+        pos.push_back(g.targets[1]);
+        break;
+    default:
+        break;
+    }
+    std::sort(pos.begin(), pos.end());
+    o.npos = int(pos.size());
+    for (size_t k = 0; k < pos.size(); ++k) o.pos[k] = pos[k];
+    return o;
+}
+
+void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
+                       RunStats& agg)
+{
+    std::vector<GpuVectorState*> live;
+    for (GpuVectorState* s : states)
+        if (!s->pending().empty()) {
+            s->prepare_batched_flush();
+            live.push_back(s);
+        }
+    if (live.empty()) return;
+    const Precision prec = live[0]->precision();
+    const TileGeom g = live[0]->geometry();
+    const int K = (prec == PREC_C64 && live[0]->fusion() > 0) ? std::min(4, std::max(3, live[0]->fusion())) : 0;
+    for (GpuVectorState* s : live)
+        if (s->precision() != prec || s->geometry().M != g.M || s->geometry().R != g.R)
+            throw Error(ERR_INVALID_ARGUMENT, "A batch needs one precision and tile shape");
+
+    // 1. plans and programs (host), small-state op lists
+    struct Plan {
+        std::vector<PlanStep> steps;
+        std::vector<size_t> poff, ooff;
+        std::vector<int> nph;
+        bool small = false;
+        int sop_begin = 0, sop_end = 0;
+        RunStats st;
+    };
+    std::vector<Plan> plans(live.size());
+    std::vector<char> host;
+    std::vector<float> blocks;
+    std::vector<SmallOp> sops;
+    std::vector<double2> svals;
+    auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
+    size_t max_steps = 0;
+    int max_small_n = 0;
+    for (size_t i = 0; i < live.size(); ++i) {
+        GpuVectorState& st = *live[i];
+        const std::vector<PendingGate>& q = st.pending();
+        Plan& p = plans[i];
+        const int n = int(st.nr_bits());
+        const double S = double(uint64_t(1) << n) * double(st.elem_bytes()) * double(st.nr_cols());
+        bool lds_ok = n < g.M + 2;
+        for (const PendingGate& pg : q)
+            if (pg.op.kind == OpKind::Dense && pg.op.targets.size() > 3) lds_ok = false;
+        if (lds_ok) {
+            p.small = true;
+            p.sop_begin = int(sops.size());
+            for (const PendingGate& pg : q)
+                if (pg.op.kind != OpKind::Identity) sops.push_back(make_small_op(pg.op, svals));
+            p.sop_end = int(sops.size());
+            p.st.sweeps = 1;
+            p.st.launches = 1;
+            p.st.gates = q.size();
+            p.st.bytes = 2.0 * S;
+            max_small_n = std::max(max_small_n, n);
+            continue;
+        }
+        p.steps = plan_gates(q, n, g, K);
+        p.poff.assign(p.steps.size(), 0);
+        p.ooff.assign(p.steps.size(), 0);
+        p.nph.assign(p.steps.size(), 0);
+        int nblocks = 0;
+        for (size_t s = 0; s < p.steps.size(); ++s) {
+            if (!p.steps[s].sweep) continue;
+            std::vector<SwPhase> ph;
+            std::vector<SwOp> ops;
+            if (K) build_fused_program(q, p.steps[s], g.M, g.R, K, ph, ops, blocks, &nblocks);
+            else build_program(q, p.steps[s], g.M, g.R, ph, ops);
+            align();
+            p.poff[s] = host.size();
+            host.insert(host.end(), reinterpret_cast<const char*>(ph.data()),
+                        reinterpret_cast<const char*>(ph.data() + ph.size()));
+            align();
+            p.ooff[s] = host.size();
+            host.insert(host.end(), reinterpret_cast<const char*>(ops.data()),
+                        reinterpret_cast<const char*>(ops.data() + ops.size()));
+            p.nph[s] = int(ph.size());
+            p.st.sweeps += 1;
+            p.st.launches += 1;
+            p.st.gates += p.steps[s].gates.size();
+            p.st.bytes += 2.0 * S;
+            p.st.phases += ph.size();
+        }
+        p.st.blocks = uint64_t(nblocks);
+        max_steps = std::max(max_steps, p.steps.size());
+    }
+    align();
+    const size_t boff = host.size();
+    host.insert(host.end(), reinterpret_cast<const char*>(blocks.data()),
+                reinterpret_cast<const char*>(blocks.data() + blocks.size()));
+    align();
+    const size_t soff = host.size();
+    host.insert(host.end(), reinterpret_cast<const char*>(sops.data()),
+                reinterpret_cast<const char*>(sops.data() + sops.size()));
+    align();
+    const size_t voff = host.size();
+    host.insert(host.end(), reinterpret_cast<const char*>(svals.data()),
+                reinterpret_cast<const char*>(svals.data() + svals.size()));
+    align();
+    // small-state items
+    std::vector<SmallItem> sitems;
+    for (size_t i = 0; i < live.size(); ++i) {
+        if (!plans[i].small || plans[i].sop_end == plans[i].sop_begin) continue;
+        for (size_t c = 0; c < live[i]->nr_cols(); ++c)
+            sitems.push_back(SmallItem{live[i]->col_ptr(c), int(live[i]->nr_bits()), plans[i].sop_begin,
+                                       plans[i].sop_end, 0});
+    }
+    const size_t sioff = host.size();
+    host.insert(host.end(), reinterpret_cast<const char*>(sitems.data()),
+                reinterpret_cast<const char*>(sitems.data() + sitems.size()));
+    align();
+    // sweep items of every round, built against the device base address
+    const size_t items_off = host.size();
+    size_t n_items_total = 0;
+    for (size_t i = 0; i < live.size(); ++i)
+        for (size_t s = 0; s < plans[i].steps.size(); ++s)
+            if (plans[i].steps[s].sweep) n_items_total += live[i]->nr_cols();
+    const size_t total = items_off + n_items_total * sizeof(SweepItem);
+    char* dbase = static_cast<char*>(prog.get(total, stream));
+    host.resize(total);
+    std::vector<std::pair<size_t, size_t>> round_items(max_steps);   // (first item, count)
+    size_t wi = 0;
+    std::vector<uint64_t> round_tiles(max_steps, 0);
+    for (size_t r = 0; r < max_steps; ++r) {
+        round_items[r].first = wi;
+        uint64_t tiles = 0;
+        for (size_t i = 0; i < live.size(); ++i) {
+            const Plan& p = plans[i];
+            if (r >= p.steps.size() || !p.steps[r].sweep) continue;
+            const int n = int(live[i]->nr_bits());
+            for (size_t c = 0; c < live[i]->nr_cols(); ++c) {
+                SweepItem it;
+                std::memset(&it, 0, sizeof(it));
+                it.amp = live[i]->col_ptr(c);
+                it.first_tile = tiles;
+                it.sa.ntiles = uint64_t(1) << (n - g.M);
+                it.sa.tset.cnt = g.M;
+                for (int j = 0; j < g.M; ++j) {
+                    it.sa.tset.s[j] = p.steps[r].tile[j];
+                    it.sa.tpos[j] = p.steps[r].tile[j];
+                }
+                it.sa.phases = reinterpret_cast<const SwPhase*>(dbase + p.poff[r]);
+                it.sa.ops = reinterpret_cast<const SwOp*>(dbase + p.ooff[r]);
+                it.sa.blocks = reinterpret_cast<const float*>(dbase + boff);
+                it.sa.nphases = p.nph[r];
+                std::memcpy(host.data() + items_off + wi * sizeof(SweepItem), &it, sizeof(it));
+                tiles += it.sa.ntiles;
+                ++wi;
+            }
+        }
+        round_items[r].second = wi - round_items[r].first;
+        round_tiles[r] = tiles;
+    }
+    SW_HIP_OK(hipMemcpyAsync(dbase, host.data(), total, hipMemcpyHostToDevice, stream));
+
+    // 2. small states: one launch
+    if (!sitems.empty()) {
+        const size_t lds = size_t(live[0]->elem_bytes()) << max_small_n;
+        if (prec == PREC_C128) {
+            SW_HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_batch<double2>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            hipLaunchKernelGGL(k_small_batch<double2>, dim3(unsigned(std::min<size_t>(sitems.size(), 65535))),
+                               dim3(256), lds, stream, reinterpret_cast<const SmallItem*>(dbase + sioff),
+                               int(sitems.size()), reinterpret_cast<const SmallOp*>(dbase + soff),
+                               reinterpret_cast<const double2*>(dbase + voff));
+        } else {
+            SW_HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_batch<float2>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            hipLaunchKernelGGL(k_small_batch<float2>, dim3(unsigned(std::min<size_t>(sitems.size(), 65535))),
+                               dim3(256), lds, stream, reinterpret_cast<const SmallItem*>(dbase + sioff),
+                               int(sitems.size()), reinterpret_cast<const SmallOp*>(dbase + soff),
+                               reinterpret_cast<const double2*>(dbase + voff));
+        }
+        SW_HIP_OK(hipGetLastError());
+        agg.launches += 1;
+    }
+
+    // 3. rounds: single-gate steps of each state, then one batched sweep launch
+    for (size_t r = 0; r < max_steps; ++r) {
+        for (size_t i = 0; i < live.size(); ++i) {
+            const Plan& p = plans[i];
+            if (r >= p.steps.size() || p.steps[r].sweep) continue;
+            for (size_t gi : p.steps[r].gates) live[i]->apply_op(live[i]->pending()[gi].op, 0, live[i]->nr_cols());
+        }
+        if (!round_items[r].second) continue;
+        const SweepItem* ditems = reinterpret_cast<const SweepItem*>(dbase + items_off) + round_items[r].first;
+        const int nit = int(round_items[r].second);
+        const unsigned gx = unsigned(std::min<uint64_t>(round_tiles[r], 4096));
+        if (prec == PREC_C128) {
+            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+        } else {
+            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+        }
+        SW_HIP_OK(hipGetLastError());
+        agg.launches += 1;
+    }
+    SW_HIP_OK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < live.size(); ++i) {
+        agg.sweeps += plans[i].st.sweeps;
+        agg.gates += plans[i].st.gates;
+        agg.bytes += plans[i].st.bytes;
+        agg.phases += plans[i].st.phases;
+        agg.blocks += plans[i].st.blocks;
+        live[i]->finish_batched_flush(plans[i].st);
+    }
 }
 
 }  // namespace q1t

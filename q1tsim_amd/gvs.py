@@ -103,6 +103,9 @@ class GpuVectorState:
     def set_plan(self, fusion_k=0, tiling=True):
         check(lib().q1t_gvs_set_plan(self.h, int(fusion_k), 1 if tiling else 0))
 
+    def set_geometry(self, M, R, L):
+        check(lib().q1t_gvs_set_geometry(self.h, int(M), int(R), int(L)))
+
     @property
     def ncols(self):
         return lib().q1t_gvs_nr_cols(self.h)
@@ -176,7 +179,7 @@ class GpuVectorState:
         s = _lib.Stats()
         check(lib().q1t_gvs_last_stats(self.h, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates, "blocks": s.blocks}
+                "gates": s.gates, "blocks": s.blocks, "phases": s.phases}
 
     def set_profiling(self, on=True):
         check(lib().q1t_gvs_set_profiling(self.h, 1 if on else 0))

@@ -298,7 +298,7 @@ class Circuit(object):
         s = _lib.Stats()
         _lib.check(self.__sim.q1t_circuit_last_stats(self.__ptr, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates, "blocks": s.blocks}
+                "gates": s.gates, "blocks": s.blocks, "phases": s.phases}
 
     def histogram_vec(self):
         n = 1 << self.nr_cbits()
@@ -306,5 +306,53 @@ class Circuit(object):
         _lib.check(self.__sim.q1t_circuit_histogram_vec(self.__ptr, out, n))
         return [int(out[i]) for i in range(n)]
 
+    @property
+    def handle(self):
+        """The circuit_t* of this circuit (for batches)."""
+        return self.__ptr
 
-__all__ = ["Circuit", "RefParam", "Q1tError"]
+
+class Batch(object):
+    """Many circuits executed together (build-added, include/q1tsim.h).
+
+    Every circuit runs as ``circuit.execute(nr_shots)`` would run it, with its
+    own seed and the same classical state; the gate runs of all circuits are
+    flushed together between their measurements.
+    """
+
+    def __init__(self, precision="c128", fusion_k=0, mem_budget=0):
+        self.__sim = _lib.lib()
+        h = ctypes.c_void_p()
+        prec = {"c128": 0, "c64": 1}[precision]
+        _lib.check(self.__sim.q1t_batch_new(prec, int(fusion_k), int(mem_budget), ctypes.byref(h)))
+        self.__ptr = h
+        self.__circuits = []
+
+    def close(self):
+        if self.__ptr:
+            self.__sim.q1t_batch_free(self.__ptr)
+            self.__ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, circuit):
+        _lib.check(self.__sim.q1t_batch_add(self.__ptr, circuit.handle))
+        self.__circuits.append(circuit)
+
+    def execute(self, nr_shots, keep_states=False):
+        _lib.check(self.__sim.q1t_batch_execute(self.__ptr, int(nr_shots), 1 if keep_states else 0))
+
+    def stats(self):
+        s = _lib.BatchStats()
+        _lib.check(self.__sim.q1t_batch_stats_get(self.__ptr, ctypes.byref(s)))
+        g = s.gates
+        return {"seconds": s.seconds, "waves": s.waves, "rounds": s.rounds, "sync_ops": s.sync_ops,
+                "bytes": g.bytes, "launches": g.launches, "sweeps": g.sweeps, "gates": g.gates,
+                "blocks": g.blocks, "phases": g.phases}
+
+
+__all__ = ["Circuit", "Batch", "RefParam", "Q1tError"]

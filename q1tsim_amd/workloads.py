@@ -69,6 +69,81 @@ def qft(nr_qbits):
     return ops
 
 
+def branching(nr_qbits, depth, nr_meas, seed):
+    """Layered random circuit (the recipe of `random_layered`) with `nr_meas`
+    mid-circuit Z measurements at distinct random layer boundaries.
+    Measurement m reads a random qubit into cbit nr_qbits + m; a classically
+    conditioned X or RZ(theta) on a random qubit follows it (control
+    [nr_qbits + m], target 1).  The circuit ends with measure_all into cbits
+    0..nr_qbits-1.  Returns (nr_cbits, program) with program entries
+    ("gate", name, qbits, params), ("measure", q, cbit, "Z"),
+    ("cgate", control, target, name, qbits, params), ("measure_all", cbits, "Z")."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    layers = []
+    for layer in range(depth):
+        ops = []
+        for q in range(nr_qbits):
+            g = ONE_QUBIT[int(rng.integers(len(ONE_QUBIT)))]
+            if g in ROTATIONS:
+                ops.append(("gate", g, [q], [float(rng.uniform(0.0, 2.0 * math.pi))]))
+            else:
+                ops.append(("gate", g, [q], []))
+        for q in range(layer % 2, nr_qbits - 1, 2):
+            ops.append(("gate", "cx", [q, q + 1], []))
+        layers.append(ops)
+    nr_meas = min(nr_meas, max(depth - 1, 0))
+    cuts = sorted(int(x) for x in rng.choice(np.arange(1, depth), size=nr_meas, replace=False)) if nr_meas else []
+    prog = []
+    m = 0
+    for layer, ops in enumerate(layers):
+        while m < len(cuts) and cuts[m] == layer:
+            q = int(rng.integers(nr_qbits))
+            prog.append(("measure", q, nr_qbits + m, "Z"))
+            t = int(rng.integers(nr_qbits))
+            if rng.uniform() < 0.5:
+                prog.append(("cgate", [nr_qbits + m], 1, "x", [t], []))
+            else:
+                prog.append(("cgate", [nr_qbits + m], 1, "rz", [t], [float(rng.uniform(0.0, 2.0 * math.pi))]))
+            m += 1
+        prog.extend(ops)
+    prog.append(("measure_all", list(range(nr_qbits)), "Z"))
+    return nr_qbits + nr_meas, prog
+
+
+def batch_workload(count, seed, qubits=(10, 22), depths=(10, 40), meas=(1, 3)):
+    """`count` branching circuits with qubit counts, depths and measurement
+    counts uniform in the given inclusive ranges.  Returns a list of
+    (nr_qbits, nr_cbits, program, circuit_seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(qubits[0], qubits[1] + 1))
+        d = int(rng.integers(depths[0], depths[1] + 1))
+        k = int(rng.integers(meas[0], meas[1] + 1))
+        cseed = int(rng.integers(1 << 62))
+        nc, prog = branching(n, d, k, cseed)
+        out.append((n, nc, prog, cseed))
+    return out
+
+
+def build_program(circ, program):
+    """Add a program (as `branching` returns it) to a circuit object with
+    q1tsim's add_gate / add_conditional_gate / measure / measure_all API."""
+    for op in program:
+        kind = op[0]
+        if kind == "gate":
+            circ.add_gate(op[1], op[2], op[3] if op[3] else None)
+        elif kind == "cgate":
+            circ.add_conditional_gate(op[1], op[2], op[3], op[4], op[5] if op[5] else None)
+        elif kind == "measure":
+            circ.measure_basis(op[1], op[2], op[3])
+        elif kind == "measure_all":
+            circ.measure_all_basis(op[1], op[2])
+        else:
+            raise ValueError("unknown program op %r" % (kind,))
+    return circ
+
+
 def unfused_bytes(ops, state_bytes):
     """Minimal traffic of each gate run on its own: 2S for dense one-qubit
     gates, RZ and X; S for S, T, CX and Swap; S/2 for CU1 (S = state bytes)."""

@@ -1,0 +1,117 @@
+"""Batched execution on the GPU: circuits of uneven width with mid-circuit
+measurements and classically conditioned gates, run through one Batch, give
+the classical state that each gives when executed alone with the same seed
+(circuit.rs:562-600 per circuit) and that the oracle's RefCircuit gives; the
+end states of the batch agree with the individually run circuits."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from q1tsim_amd import workloads
+from q1tsim_amd.q1tsim import Batch, Circuit
+
+pytestmark = pytest.mark.gpu
+
+SHOTS = 64
+
+
+def make(n, nc, prog, seed, prec, fusion=0):
+    c = Circuit(n, nc)
+    workloads.build_program(c, prog)
+    c.set_backend(prec, fusion)
+    c.seed(seed)
+    return c
+
+
+def make_ref(n, nc, prog):
+    r = orc.RefCircuit(n, nc)
+    for op in prog:
+        if op[0] == "gate":
+            r.add_gate(op[1], op[2], op[3])
+        elif op[0] == "cgate":
+            r.add_conditional_gate(op[1], op[2], op[3], op[4], op[5])
+        elif op[0] == "measure":
+            r.measure(op[1], op[2], op[3])
+        elif op[0] == "measure_all":
+            r.measure_all(op[1], op[2])
+    return r
+
+
+def run_batch(wl, prec, fusion=0, budget=0, keep=False):
+    batch = Batch(prec, fusion, budget)
+    circs = [make(n, nc, prog, s, prec, fusion) for (n, nc, prog, s) in wl]
+    for c in circs:
+        batch.add(c)
+    batch.execute(SHOTS, keep_states=keep)
+    return batch, circs
+
+
+# widths 3..16 cover both batch paths: whole-state-in-LDS (n <= 13) and
+# tiled sweeps over all states' tiles in one launch (n >= 14)
+WL = workloads.batch_workload(24, seed=11, qubits=(3, 16), depths=(3, 12), meas=(1, 3))
+
+
+def test_batch_cstate_c128_matches_solo_and_oracle():
+    batch, circs = run_batch(WL, "c128")
+    st = batch.stats()
+    assert st["waves"] == 1 and st["sync_ops"] > 0 and st["gates"] > 0
+    for (n, nc, prog, s), c in zip(WL, circs):
+        solo = make(n, nc, prog, s, "c128")
+        solo.execute(SHOTS)
+        got = np.array(c.cstate(), dtype=np.uint64)
+        assert np.array_equal(got, np.array(solo.cstate(), dtype=np.uint64)), (n, s)
+        if n <= 12:
+            r = make_ref(n, nc, prog)
+            r.execute(SHOTS, s)
+            assert np.array_equal(got, r.cstate()), (n, s)
+
+
+def test_batch_end_states_c128():
+    wl = [(n, nc, prog[:-1], s) for (n, nc, prog, s) in WL]   # stop before measure_all
+    _, circs = run_batch(wl, "c128", keep=True)
+    for (n, nc, prog, s), c in zip(wl, circs):
+        solo = make(n, nc, prog, s, "c128")
+        solo.execute(SHOTS)
+        assert c.counts() == solo.counts(), (n, s)
+        for col in range(c.nr_columns()):
+            a, b = c.amplitudes(col), solo.amplitudes(col)
+            assert np.max(np.abs(a - b)) <= 1e-12, (n, s, col)
+
+
+@pytest.mark.parametrize("fusion", [0, 4])
+def test_batch_c64(fusion):
+    wl = [(n, nc, prog[:-1], s) for (n, nc, prog, s) in WL]
+    _, circs = run_batch(wl, "c64", fusion=fusion, keep=True)
+    for (n, nc, prog, s), c in zip(wl, circs):
+        ref = make(n, nc, prog, s, "c128")
+        ref.execute(SHOTS)
+        # branch outcomes come from FP64 sums of FP32 amplitudes: a flip
+        # needs a uniform within ~1e-7 of a marginal, absent for these seeds
+        assert c.counts() == ref.counts(), (n, s)
+        for col in range(c.nr_columns()):
+            a, b = c.amplitudes(col), ref.amplitudes(col)
+            fid = abs(np.vdot(b, a)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+            assert 1.0 - fid <= 1e-6, (n, s, col, 1.0 - fid)
+
+
+def test_batch_waves_under_small_budget():
+    wl = WL[:10]
+    big, circs_big = run_batch(wl, "c128")
+    small, circs_small = run_batch(wl, "c128", budget=1 << 20)
+    assert small.stats()["waves"] > 1 and big.stats()["waves"] == 1
+    for a, b in zip(circs_big, circs_small):
+        assert np.array_equal(np.array(a.cstate()), np.array(b.cstate()))
+
+
+def test_batch_same_circuit_many_seeds():
+    # one program, 32 seeds: every copy's histogram is the one it gives alone
+    n, nc, prog, _ = WL[0]
+    wl = [(n, nc, prog, 1000 + k) for k in range(32)]
+    _, circs = run_batch(wl, "c128")
+    total = 0
+    for (n_, nc_, prog_, s), c in zip(wl, circs):
+        solo = make(n_, nc_, prog_, s, "c128")
+        solo.execute(SHOTS)
+        assert c.histogram() == solo.histogram()
+        total += sum(c.histogram_vec())
+    assert total == 32 * SHOTS
