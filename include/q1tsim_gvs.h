@@ -162,6 +162,42 @@ q1t_xoshiro* q1t_xoshiro_new(uint64_t seed);
 void q1t_xoshiro_free(q1t_xoshiro* x);
 uint64_t q1t_xoshiro_next(void* ctx);
 
+/* ---- learned sweep cost model (no reference counterpart: q1tsim applies
+ * gates one by one, src/circuit.rs:643-762, and has no planner) ----
+ * A 16 -> 64 -> 64 -> 1 tanh MLP in FP32 on standardised features that
+ * predicts log(microseconds per 1024 tiles) of a sweep program.  Training
+ * and inference run in HIP kernels; the model only ranks equivalent
+ * programs, it never changes amplitudes. */
+#define Q1T_CM_FEATURES 16
+#define Q1T_CM_PARAMS 5313
+typedef struct q1t_cost_model q1t_cost_model;
+int q1t_cm_new(uint64_t seed, q1t_cost_model** out);
+void q1t_cm_free(q1t_cost_model* cm);
+int q1t_cm_get_params(const q1t_cost_model* cm, float* out, size_t len);
+int q1t_cm_set_params(q1t_cost_model* cm, const float* p, size_t len);
+/* x' = (x - mu) / sigma per feature, sigma > 0 */
+int q1t_cm_set_norm(q1t_cost_model* cm, const float* mu, const float* sigma, size_t nfeat);
+int q1t_cm_get_norm(const q1t_cost_model* cm, float* mu, float* sigma, size_t nfeat);
+int q1t_cm_predict(q1t_cost_model* cm, const float* x, size_t nrows, float* y);
+/* mean squared error and its gradient (layout of the parameters) */
+int q1t_cm_loss_grad(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, float* grad, size_t len,
+                     double* loss);
+int q1t_cm_adam_step(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, float lr, double* loss);
+/* minibatch Adam over the set; *loss = final full-set loss; marks the model trained */
+int q1t_cm_train(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, int epochs, size_t batch,
+                 float lr, uint64_t seed, double* loss);
+int q1t_cm_set_trained(q1t_cost_model* cm, int trained);
+
+/* planner mode of a state's fused complex64 flushes: 0 greedy fixed-width
+ * fusion, 1 learned (cm ranks the candidate programs; greedy while cm is
+ * untrained or NULL), 2 seeded random choice among the candidates */
+int q1t_gvs_set_cost_model(q1t_gvs* s, q1t_cost_model* cm, int mode, uint64_t seed);
+/* record every sweep's features and measured time (turns profiling on) */
+int q1t_gvs_set_recording(q1t_gvs* s, int on);
+/* move up to cap recorded samples out, oldest first (the rest stay):
+ * feat[cap][16], tiles[cap] (tiles over all columns), ms[cap] */
+int q1t_gvs_take_samples(q1t_gvs* s, float* feat, double* tiles, double* ms, size_t cap, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

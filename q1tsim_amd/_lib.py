@@ -48,6 +48,7 @@ class Rng(ctypes.Structure):
 _vp, _sz, _u64, _i32, _d = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
 _ps = ctypes.POINTER(ctypes.c_size_t)
 _pd = ctypes.POINTER(ctypes.c_double)
+_pf = ctypes.POINTER(ctypes.c_float)
 _pu64 = ctypes.POINTER(ctypes.c_uint64)
 _pu8 = ctypes.POINTER(ctypes.c_uint8)
 _cs = ctypes.c_char_p
@@ -82,6 +83,20 @@ SIGNATURES = {
     "q1t_circuit_read_amplitudes": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_circuit_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_circuit_histogram_vec": (_i32, [_vp, _ps, _sz]),
+    "q1t_cm_new": (_i32, [_u64, ctypes.POINTER(_vp)]),
+    "q1t_cm_free": (None, [_vp]),
+    "q1t_cm_get_params": (_i32, [_vp, _pf, _sz]),
+    "q1t_cm_set_params": (_i32, [_vp, _pf, _sz]),
+    "q1t_cm_set_norm": (_i32, [_vp, _pf, _pf, _sz]),
+    "q1t_cm_get_norm": (_i32, [_vp, _pf, _pf, _sz]),
+    "q1t_cm_predict": (_i32, [_vp, _pf, _sz, _pf]),
+    "q1t_cm_loss_grad": (_i32, [_vp, _pf, _pf, _sz, _pf, _sz, _pd]),
+    "q1t_cm_adam_step": (_i32, [_vp, _pf, _pf, _sz, ctypes.c_float, _pd]),
+    "q1t_cm_train": (_i32, [_vp, _pf, _pf, _sz, _i32, _sz, ctypes.c_float, _u64, _pd]),
+    "q1t_cm_set_trained": (_i32, [_vp, _i32]),
+    "q1t_gvs_set_cost_model": (_i32, [_vp, _vp, _i32, _u64]),
+    "q1t_gvs_set_recording": (_i32, [_vp, _i32]),
+    "q1t_gvs_take_samples": (_i32, [_vp, _pf, _pd, _pd, _sz, _ps]),
     "q1t_batch_new": (_i32, [_i32, _i32, _sz, ctypes.POINTER(_vp)]),
     "q1t_batch_free": (None, [_vp]),
     "q1t_batch_add": (_i32, [_vp, _vp]),

@@ -17,11 +17,15 @@ ARCH = "gfx950"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           "-I", os.path.join(os.path.dirname(HERE), "include")]
-# Packed FP32 (v_pk_*_f32) stays off for all device code.
+# Packed FP32 (v_pk_*_f32) stays off for all device code.  Sinking of
+# common code out of switch cases is off: it merges the per-register-slot
+# cases of the sweep kernels into one body with a run-time index into the
+# register array, which puts that array in scratch memory.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics", "-mllvm", "-simplifycfg-sink-common=false"]
 
-SOURCES = ["gvs.hip", "sweep.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp", "capi.cpp"]
+SOURCES = ["gvs.hip", "sweep.hip", "costmodel.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp",
+           "capi.cpp"]
 
 
 def _compile(src):

@@ -5,9 +5,11 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "../../include/q1tsim.h"
+#include "q1tsim.h"
 #include "batch.h"
 #include "circuit.h"
+#include "costmodel.h"
+#include "sweep.h"
 
 using namespace q1t;
 
@@ -699,5 +701,161 @@ extern "C" int q1t_gate_matrix(const char* name, const double* params, size_t np
             out[2 * k + 1] = m[k].imag();
         }
         *nbits = gate_nbits(*g);
+    });
+}
+
+// ------------------------------------------------- learned cost model
+
+struct q1t_cost_model {
+    CostModel m;
+};
+
+static_assert(kCmParams == Q1T_CM_PARAMS, "parameter count of the C ABI");
+static_assert(kCmFeat == Q1T_CM_FEATURES && kSweepFeatures == Q1T_CM_FEATURES, "feature count of the C ABI");
+
+extern "C" int q1t_cm_new(uint64_t seed, q1t_cost_model** out)
+{
+    NEED(out);
+    *out = nullptr;
+    return guard([&] {
+        q1t_cost_model* cm = new q1t_cost_model();
+        cm->m.init(seed);
+        *out = cm;
+    });
+}
+
+extern "C" void q1t_cm_free(q1t_cost_model* cm) { delete cm; }
+
+extern "C" int q1t_cm_get_params(const q1t_cost_model* cm, float* out, size_t len)
+{
+    NEED(cm);
+    NEED(out);
+    return guard([&] {
+        if (len < size_t(kCmParams)) throw err_not_enough_space(len, kCmParams);
+        cm->m.get_params(out);
+    });
+}
+
+extern "C" int q1t_cm_set_params(q1t_cost_model* cm, const float* p, size_t len)
+{
+    NEED(cm);
+    NEED(p);
+    return guard([&] {
+        if (len != size_t(kCmParams)) throw Error(ERR_INVALID_ARGUMENT, "Expected " + std::to_string(kCmParams) + " parameters");
+        cm->m.set_params(p);
+    });
+}
+
+extern "C" int q1t_cm_set_norm(q1t_cost_model* cm, const float* mu, const float* sigma, size_t nfeat)
+{
+    NEED(cm);
+    NEED(mu);
+    NEED(sigma);
+    return guard([&] {
+        if (nfeat != size_t(kCmFeat)) throw Error(ERR_INVALID_ARGUMENT, "Expected 16 features");
+        cm->m.set_norm(mu, sigma);
+    });
+}
+
+extern "C" int q1t_cm_get_norm(const q1t_cost_model* cm, float* mu, float* sigma, size_t nfeat)
+{
+    NEED(cm);
+    NEED(mu);
+    NEED(sigma);
+    return guard([&] {
+        if (nfeat < size_t(kCmFeat)) throw err_not_enough_space(nfeat, kCmFeat);
+        cm->m.get_norm(mu, sigma);
+    });
+}
+
+extern "C" int q1t_cm_predict(q1t_cost_model* cm, const float* x, size_t nrows, float* y)
+{
+    NEED(cm);
+    NEED(x);
+    NEED(y);
+    return guard([&] { cm->m.predict(x, nrows, y); });
+}
+
+extern "C" int q1t_cm_loss_grad(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, float* grad,
+                                size_t len, double* loss)
+{
+    NEED(cm);
+    NEED(x);
+    NEED(t);
+    NEED(grad);
+    NEED(loss);
+    return guard([&] {
+        if (len < size_t(kCmParams)) throw err_not_enough_space(len, kCmParams);
+        *loss = cm->m.loss_and_grad(x, t, nrows, grad);
+    });
+}
+
+extern "C" int q1t_cm_adam_step(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, float lr,
+                                double* loss)
+{
+    NEED(cm);
+    NEED(x);
+    NEED(t);
+    NEED(loss);
+    return guard([&] {
+        AdamConfig c;
+        c.lr = lr;
+        *loss = cm->m.adam_step(x, t, nrows, c);
+    });
+}
+
+extern "C" int q1t_cm_train(q1t_cost_model* cm, const float* x, const float* t, size_t nrows, int epochs,
+                            size_t batch, float lr, uint64_t seed, double* loss)
+{
+    NEED(cm);
+    NEED(x);
+    NEED(t);
+    NEED(loss);
+    return guard([&] {
+        if (epochs < 0) throw Error(ERR_INVALID_ARGUMENT, "epochs must be >= 0");
+        AdamConfig c;
+        c.lr = lr;
+        *loss = cm->m.train(x, t, nrows, epochs, batch, c, seed);
+    });
+}
+
+extern "C" int q1t_cm_set_trained(q1t_cost_model* cm, int trained)
+{
+    NEED(cm);
+    cm->m.mark_trained(trained != 0);
+    return 0;
+}
+
+extern "C" int q1t_gvs_set_cost_model(q1t_gvs* s, q1t_cost_model* cm, int mode, uint64_t seed)
+{
+    NEED(s);
+    return guard([&] {
+        if (mode < PLAN_GREEDY || mode > PLAN_EXPLORE) throw Error(ERR_INVALID_ARGUMENT, "Unknown planner mode");
+        s->s.set_cost_model(cm ? &cm->m : nullptr, mode, seed);
+    });
+}
+
+extern "C" int q1t_gvs_set_recording(q1t_gvs* s, int on)
+{
+    NEED(s);
+    return guard([&] { s->s.set_recording(on != 0); });
+}
+
+extern "C" int q1t_gvs_take_samples(q1t_gvs* s, float* feat, double* tiles, double* ms, size_t cap, size_t* count)
+{
+    NEED(s);
+    NEED(feat);
+    NEED(tiles);
+    NEED(ms);
+    NEED(count);
+    return guard([&] {
+        std::vector<SweepSample> v = s->s.take_samples(cap);
+        const size_t n = v.size();
+        for (size_t i = 0; i < n; ++i) {
+            std::memcpy(feat + i * kCmFeat, v[i].feat, kCmFeat * sizeof(float));
+            tiles[i] = v[i].tiles;
+            ms[i] = v[i].ms;
+        }
+        *count = n;
     });
 }

@@ -44,6 +44,23 @@ struct StateComparison {
     double overlap_re = 0, overlap_im = 0, max_abs = 0, norm_a = 0, norm_b = 0;
 };
 
+class CostModel;
+
+// How the fused complex64 planner picks among equivalent sweep programs.
+enum PlanMode {
+    PLAN_GREEDY = 0,    // fixed-width greedy fusion (blocks of the configured width)
+    PLAN_LEARNED = 1,   // candidate programs ranked by the learned cost model
+    PLAN_EXPLORE = 2    // seeded random choice among the candidates (data collection)
+};
+
+// One recorded sweep: program features, tiles processed, measured time.
+struct SweepSample {
+    float feat[16];
+    double tiles = 0.0;   // tiles over all columns of the launch
+    int slot = -1;        // profiling slot of the launch until its time is read
+    double ms = 0.0;
+};
+
 struct PendingGate {
     GateOp op;
     std::vector<cdouble> matrix;  // full FP64 matrix (fused path composes these)
@@ -125,6 +142,23 @@ public:
     void* scratch_matrix(size_t bytes) { return scratch(SLOT_MATRIX, bytes); }
     void apply_op(const GateOp& op, size_t col_begin, size_t col_end);
 
+    // planner: learned cost model (not owned) and mode; recording of sweeps
+    void set_cost_model(CostModel* cm, int mode, uint64_t seed)
+    {
+        cost_model_ = cm;
+        plan_mode_ = mode;
+        plan_rng_.reseed(seed);
+    }
+    CostModel* cost_model() const { return cost_model_; }
+    int plan_mode() const { return plan_mode_; }
+    Xoshiro256& plan_rng() { return plan_rng_; }
+    void set_recording(bool on);
+    bool recording() const { return recording_; }
+    void add_sample(const float* feat, double tiles, int slot);
+    // up to `max` finished samples, oldest first (the rest stay)
+    std::vector<SweepSample> take_samples(size_t max);
+    size_t sample_count() const { return samples_done_; }
+
     // kernel profiling (off by default): every launch of a flush is bracketed
     // by HIP events; the times are gathered after the flush synchronises.
     void set_profiling(bool on);
@@ -159,7 +193,15 @@ private:
     void* d_slot_[SLOT_COUNT] = {nullptr, nullptr, nullptr};
     size_t slot_cap_[SLOT_COUNT] = {0, 0, 0};
 
+    CostModel* cost_model_ = nullptr;
+    int plan_mode_ = PLAN_GREEDY;
+    Xoshiro256 plan_rng_{1};
+    bool recording_ = false;
+    std::vector<SweepSample> samples_;
+    size_t samples_done_ = 0;             // samples whose time is known
+
     bool profiling_ = false;
+    std::vector<float> slot_ms_;          // per profiling slot, last collection
     struct PendingEvent {
         int pair;
         std::string name;

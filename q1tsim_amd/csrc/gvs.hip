@@ -1233,13 +1233,39 @@ void GpuVectorState::prof_end(int slot, const char* name, double bytes)
     ev_pending_.push_back(PendingEvent{slot, name, bytes});
 }
 
+void GpuVectorState::set_recording(bool on)
+{
+    recording_ = on;
+    if (on) set_profiling(true);
+}
+
+void GpuVectorState::add_sample(const float* feat, double tiles, int slot)
+{
+    SweepSample s;
+    std::memcpy(s.feat, feat, sizeof(s.feat));
+    s.tiles = tiles;
+    s.slot = slot;
+    samples_.push_back(s);
+}
+
+std::vector<SweepSample> GpuVectorState::take_samples(size_t max)
+{
+    const std::ptrdiff_t n = std::ptrdiff_t(std::min(max, samples_done_));
+    std::vector<SweepSample> out(samples_.begin(), samples_.begin() + n);
+    samples_.erase(samples_.begin(), samples_.begin() + n);
+    samples_done_ -= size_t(n);
+    return out;
+}
+
 void GpuVectorState::prof_collect()
 {
     if (ev_pending_.empty()) return;
     HIP_OK(hipStreamSynchronize(stream_));
+    slot_ms_.assign(ev_used_, 0.0f);
     for (const PendingEvent& pe : ev_pending_) {
         float ms = 0.0f;
         HIP_OK(hipEventElapsedTime(&ms, ev_pool_[pe.pair].first, ev_pool_[pe.pair].second));
+        slot_ms_[size_t(pe.pair)] = ms;
         KernelProfile* kp = nullptr;
         for (KernelProfile& k : prof_)
             if (k.name == pe.name) kp = &k;
@@ -1251,6 +1277,12 @@ void GpuVectorState::prof_collect()
         kp->ms += ms;
         kp->bytes += pe.bytes;
     }
+    for (size_t i = samples_done_; i < samples_.size(); ++i) {
+        SweepSample& sm = samples_[i];
+        if (sm.slot >= 0 && size_t(sm.slot) < slot_ms_.size()) sm.ms = slot_ms_[size_t(sm.slot)];
+        sm.slot = -1;
+    }
+    samples_done_ = samples_.size();
     ev_pending_.clear();
     ev_used_ = 0;
 }

@@ -194,19 +194,38 @@ uint32_t dense_form(const cdouble* m)
     return SWF_GENERAL;
 }
 
+// Where local bit b lives in a phase with register bits `regs`: register
+// index (number of register bits below b) or thread-index bit (number of
+// non-register bits below b).
+static void local_slot(uint32_t regs, int b, bool* in_reg, int* idx)
+{
+    int r = 0, t = 0;
+    for (int c = 0; c < b; ++c) ((regs >> c) & 1) ? ++r : ++t;
+    *in_reg = (regs >> b) & 1;
+    *idx = *in_reg ? r : t;
+}
+
 SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 {
     SwOp o;
     std::memset(&o, 0, sizeof(o));
-    o.dlocal = -1;
+    o.dreg = -1;
+    o.dtid = -1;
     o.dglobal = -1;
-    lm.split(op.ctrl_mask, o.cl, o.co);
+    uint32_t cl = 0;
+    lm.split(op.ctrl_mask, cl, o.co);
+    for (int b = 0; b < 32; ++b)
+        if ((cl >> b) & 1) {
+            bool in_reg;
+            int idx;
+            local_slot(regs, b, &in_reg, &idx);
+            if (in_reg) o.cr |= 1u << idx;
+            else o.ct |= 1u << idx;
+        }
     if (op.kind == OpKind::Dense) {
         o.type = SW_DENSE1;
-        const int lb = lm.local_of[op.targets[0]];
-        int idx = 0;
-        for (int b = 0; b < lb; ++b) idx += (regs >> b) & 1;
-        o.reg = idx;
+        bool in_reg;
+        local_slot(regs, lm.local_of[op.targets[0]], &in_reg, &o.reg);
         for (int k = 0; k < 4; ++k) {
             o.m[2 * k] = op.m[k].real();
             o.m[2 * k + 1] = op.m[k].imag();
@@ -221,8 +240,14 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
     } else {  // Diag on one target
         o.type = SW_DIAG1;
         const int t = op.targets[0];
-        if (lm.local_of[t] >= 0) o.dlocal = lm.local_of[t];
-        else o.dglobal = t;
+        if (lm.local_of[t] >= 0) {
+            bool in_reg;
+            int idx;
+            local_slot(regs, lm.local_of[t], &in_reg, &idx);
+            (in_reg ? o.dreg : o.dtid) = idx;
+        } else {
+            o.dglobal = t;
+        }
         o.m[0] = op.m[0].real();
         o.m[1] = op.m[0].imag();
         o.m[2] = op.m[1].real();
@@ -427,6 +452,45 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
         if (nblocks) ++*nblocks;
         pending.swap(rest);
     }
+}
+
+void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, const PlanStep& step,
+                    int precision, const TileGeom& g, float* f)
+{
+    for (int k = 0; k < kSweepFeatures; ++k) f[k] = 0.0f;
+    for (const SwPhase& ph : phases) {
+        int nb;
+        if (ph.kind == 1) {
+            nb = int(ph.reg_mask);
+            f[nb == 3 ? 1 : 2] += 1.0f;
+        } else {
+            nb = g.R;
+            f[0] += 1.0f;
+        }
+        for (int j = 0; j < nb; ++j)
+            if (ph.rpos[j] >= 8) {
+                f[13] += 1.0f;
+                break;
+            }
+    }
+    for (const SwOp& o : ops) {
+        if (o.type == SW_DENSE1) {
+            if (o.aux == SWF_GENERAL) f[3] += 1.0f;
+            else if (o.aux == SWF_SWAP) f[5] += 1.0f;
+            else f[4] += 1.0f;
+        } else {
+            f[6] += 1.0f;
+        }
+        if (o.ct) f[7] += 1.0f;
+        if (o.cr) f[8] += 1.0f;
+        if (o.co) f[9] += 1.0f;
+    }
+    f[10] = float(precision);
+    f[11] = float(g.M);
+    f[12] = float(g.R);
+    for (int b : step.tile)
+        if (b >= 20) f[14] += 1.0f;
+    f[15] = float(step.gates.size());
 }
 
 }  // namespace q1t

@@ -16,13 +16,19 @@ enum SwDenseForm : uint32_t { SWF_GENERAL = 0, SWF_REAL = 1, SWF_SWAP = 2, SWF_R
 // Body of a SW_PHASE op (SwOp::aux): general, times i, times -i, times -1.
 enum SwPhaseForm : uint32_t { SWP_GENERAL = 0, SWP_I = 1, SWP_MINUS_I = 2, SWP_NEG = 3 };
 
+// Controls (and the ones-mask of a phase) are split by where their bits live
+// during the op's phase: `cr` over register indices (uniform per register
+// slot), `ct` over thread-index bits (one lane test per op), `co` over global
+// bits outside the tile (uniform per tile).
 struct SwOp {
     int32_t type;
     int32_t reg;       // SW_DENSE1: index of the target among the phase's register bits
-    int32_t dlocal;    // SW_DIAG1: local bit of the target, or -1
+    int32_t dreg;      // SW_DIAG1: register index of the target, or -1
+    int32_t dtid;      // SW_DIAG1: thread-index bit of the target, or -1
     int32_t dglobal;   // SW_DIAG1: global bit of the target when it is outside the tile, or -1
-    uint32_t cl;       // control (or phase) mask over local tile bits
-    uint32_t aux;      // SW_BLOCK: offset of the block matrix in the block table
+    uint32_t cr;       // control (or phase) mask over register indices
+    uint32_t ct;       // control (or phase) mask over thread-index bits
+    uint32_t aux;      // SW_DENSE1 / SW_PHASE: body form; SW_BLOCK: offset of the block matrix
     uint64_t co;       // control (or phase) mask over global bits outside the tile
     double m[8];       // SW_DENSE1: m00, m01, m10, m11 (re, im); SW_PHASE: m[0..1]; SW_DIAG1: d0, d1
 };
@@ -65,6 +71,15 @@ void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int 
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
                          int* nblocks);
+
+// Features of a sweep program for the learned cost model (16 floats):
+// register phases, blocks of 3 and of 4 qubits, one-qubit ops by body
+// (general, real/RX/Y, swap, phase/diagonal), ops with thread, register and
+// outer controls, precision, M, R, phases holding a local bit >= 8 in
+// registers, tile bits at positions >= 20 (long HBM strides), gates.
+constexpr int kSweepFeatures = 16;
+void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, const PlanStep& step,
+                    int precision, const TileGeom& g, float* out);
 
 // Device buffer that grows on demand (synchronising `stream` first).
 struct DeviceBuffer {

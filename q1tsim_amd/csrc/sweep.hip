@@ -12,10 +12,13 @@
 #include "sweep.h"
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <set>
 
+#include "costmodel.h"
 #include "kernels.h"
 
 namespace q1t {
@@ -48,52 +51,118 @@ template <int SW, int M> __device__ __forceinline__ int swz(int l)
     return l ^ (f & ((1 << SW) - 1));
 }
 
-// One-qubit update of the register pairs (k, k | 2^J) whose control bits are
-// set.  FORM picks the body from the matrix structure, as the reference's
-// per-gate apply_mat_slice bodies do: general complex, real (H, RY), swap
-// (X, CX), [[a, ib], [ib, a]] (RX) and [[0, -i], [i, 0]] (Y).
-template <typename V, int NR, int J, int FORM>
-__device__ __forceinline__ void dense_pairs(V (&v)[NR], const uint32_t (&lidx)[NR], uint32_t cl, V m0, V m1, V m2, V m3)
+// c ? a : b per component (a conditional on whole vector structs selects
+// between their addresses, which keeps register arrays out of registers).
+template <typename V> __device__ __forceinline__ V sel(bool c, V a, V b)
+{
+    return mkc<V>(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// One-qubit update of the register pairs (k, k | 2^J) whose register
+// controls `cr` are set; with TC the lane's thread controls decide (`tok`).
+// FORM picks the body from the matrix structure, as the reference's per-gate
+// apply_mat_slice bodies do: general complex, real (H, RY), swap (X, CX),
+// [[a, ib], [ib, a]] (RX) and [[0, -i], [i, 0]] (Y).
+template <typename V, int NR, int J, int FORM, bool TC>
+__device__ __forceinline__ void dense_pairs(V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         if ((k >> J) & 1) continue;
+        if ((uint32_t(k) & cr) != cr) continue;
         const int k1 = k | (1 << J);
-        if ((lidx[k] & cl) == cl) {
-            const V x = v[k], y = v[k1];
-            if constexpr (FORM == SWF_GENERAL) {
-                v[k] = cadd(cmul(m0, x), cmul(m1, y));
-                v[k1] = cadd(cmul(m2, x), cmul(m3, y));
-            } else if constexpr (FORM == SWF_REAL) {
-                v[k] = mkc<V>(m0.x * x.x + m1.x * y.x, m0.x * x.y + m1.x * y.y);
-                v[k1] = mkc<V>(m2.x * x.x + m3.x * y.x, m2.x * x.y + m3.x * y.y);
-            } else if constexpr (FORM == SWF_SWAP) {
-                v[k] = y;
-                v[k1] = x;
-            } else if constexpr (FORM == SWF_RX) {
-                const auto a = m0.x, b = m1.y;
-                v[k] = mkc<V>(a * x.x - b * y.y, a * x.y + b * y.x);
-                v[k1] = mkc<V>(a * y.x - b * x.y, a * y.y + b * x.x);
-            } else {  // SWF_Y
-                v[k] = mkc<V>(y.y, -y.x);
-                v[k1] = mkc<V>(-x.y, x.x);
-            }
+        const V x = v[k], y = v[k1];
+        V nx, ny;
+        if constexpr (FORM == SWF_GENERAL) {
+            nx = cadd(cmul(m0, x), cmul(m1, y));
+            ny = cadd(cmul(m2, x), cmul(m3, y));
+        } else if constexpr (FORM == SWF_REAL) {
+            nx = mkc<V>(m0.x * x.x + m1.x * y.x, m0.x * x.y + m1.x * y.y);
+            ny = mkc<V>(m2.x * x.x + m3.x * y.x, m2.x * x.y + m3.x * y.y);
+        } else if constexpr (FORM == SWF_SWAP) {
+            nx = y;
+            ny = x;
+        } else if constexpr (FORM == SWF_RX) {
+            const auto a = m0.x, b = m1.y;
+            nx = mkc<V>(a * x.x - b * y.y, a * x.y + b * y.x);
+            ny = mkc<V>(a * y.x - b * x.y, a * y.y + b * x.x);
+        } else {  // SWF_Y
+            nx = mkc<V>(y.y, -y.x);
+            ny = mkc<V>(-x.y, x.x);
+        }
+        if constexpr (TC) {
+            v[k] = sel(tok, nx, x);
+            v[k1] = sel(tok, ny, y);
+        } else {
+            v[k] = nx;
+            v[k1] = ny;
         }
     }
 }
 
-template <typename V, int NR, int R, int FORM>
-__device__ __forceinline__ void dense_reg(int reg, V (&v)[NR], const uint32_t (&lidx)[NR], uint32_t cl, V m0, V m1,
-                                          V m2, V m3)
+template <typename V, int NR, int R, int FORM, bool TC>
+__device__ __forceinline__ void dense_reg(int reg, V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
     switch (reg) {
-    case 0: dense_pairs<V, NR, 0, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
-    case 1: dense_pairs<V, NR, 1, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
-    case 2: dense_pairs<V, NR, 2, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
-    case 3: dense_pairs<V, NR, 3, FORM>(v, lidx, cl, m0, m1, m2, m3); break;
+    case 0: dense_pairs<V, NR, 0, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 1: dense_pairs<V, NR, 1, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 2: dense_pairs<V, NR, 2, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 3: dense_pairs<V, NR, 3, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
     default:
-        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM>(v, lidx, cl, m0, m1, m2, m3);
+        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM, TC>(v, cr, tok, m0, m1, m2, m3);
         break;
+    }
+}
+
+template <typename V, int NR, int R, bool TC>
+__device__ __forceinline__ void dense_form_dispatch(uint32_t form, int reg, V (&v)[NR], uint32_t cr, bool tok, V m0,
+                                                    V m1, V m2, V m3)
+{
+    switch (form) {
+    case SWF_REAL: dense_reg<V, NR, R, SWF_REAL, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    case SWF_SWAP: dense_reg<V, NR, R, SWF_SWAP, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    case SWF_RX: dense_reg<V, NR, R, SWF_RX, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    case SWF_Y: dense_reg<V, NR, R, SWF_Y, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    default: dense_reg<V, NR, R, SWF_GENERAL, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    }
+}
+
+// Multiply the registers whose register controls `cr` are set (and, with TC,
+// whose lane passes `tok`) by the value f(k, v[k]) gives.
+template <typename V, int NR, bool TC, typename F>
+__device__ __forceinline__ void scale_regs(V (&v)[NR], uint32_t cr, bool tok, F f)
+{
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if ((uint32_t(k) & cr) != cr) continue;
+        const V nv = f(k, v[k]);
+        if constexpr (TC) v[k] = sel(tok, nv, v[k]);
+        else v[k] = nv;
+    }
+}
+
+template <typename V, int NR, bool TC>
+__device__ __forceinline__ void phase_op(uint32_t form, V (&v)[NR], uint32_t cr, bool tok, V ph1)
+{
+    switch (form) {
+    case SWP_I: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(-x.y, x.x); }); break;
+    case SWP_MINUS_I: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(x.y, -x.x); }); break;
+    case SWP_NEG: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(-x.x, -x.y); }); break;
+    default: scale_regs<V, NR, TC>(v, cr, tok, [&](int, V x) { return cmul(ph1, x); }); break;
+    }
+}
+
+template <typename V, int NR, bool TC>
+__device__ __forceinline__ void diag_op(const SwOp& op, V (&v)[NR], uint32_t cr, bool tok, uint64_t base, int tid,
+                                        V d0, V d1)
+{
+    if (op.dreg >= 0) {
+        const int dr = op.dreg;
+        scale_regs<V, NR, TC>(v, cr, tok, [&](int k, V x) { return cmul(sel(((k >> dr) & 1) != 0, d1, d0), x); });
+    } else {
+        const bool one = op.dtid >= 0 ? ((tid >> op.dtid) & 1) : ((base >> op.dglobal) & 1);
+        const V d = sel(one, d1, d0);
+        scale_regs<V, NR, TC>(v, cr, tok, [&](int, V x) { return cmul(d, x); });
     }
 }
 
@@ -208,74 +277,41 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
 #pragma unroll
             for (int b = 0; b < M - R; ++b)
                 if ((tid >> b) & 1) tpart |= 1u << ph.tbit[b];
-            uint32_t rbit[R];
-#pragma unroll
-            for (int b = 0; b < R; ++b) rbit[b] = 1u << ph.rpos[b];
-            uint32_t lidx[NR];
-            V v[NR];
+            uint32_t rp[NR];   // wave-uniform local offsets of the register slots
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                uint32_t l = tpart;
+                uint32_t l = 0;
 #pragma unroll
                 for (int b = 0; b < R; ++b)
-                    if ((k >> b) & 1) l |= rbit[b];
-                lidx[k] = l;
-                v[k] = lds[swz<SW, M>(int(l))];
+                    if ((k >> b) & 1) l |= 1u << ph.rpos[b];
+                rp[k] = l;
             }
+            V v[NR];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) v[k] = lds[swz<SW, M>(int(tpart | rp[k]))];
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
                 const SwOp& op = sa.ops[oi];
                 if ((base & op.co) != op.co) continue;
+                const uint32_t cr = op.cr;
+                const bool tc = op.ct != 0;
+                const bool tok = (uint32_t(tid) & op.ct) == op.ct;
                 if (op.type == SW_DENSE1) {
                     const V m0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), m1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
                     const V m2 = mkc<V>(Rl(op.m[4]), Rl(op.m[5])), m3 = mkc<V>(Rl(op.m[6]), Rl(op.m[7]));
-                    switch (op.aux) {
-                    case SWF_REAL: dense_reg<V, NR, R, SWF_REAL>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case SWF_SWAP: dense_reg<V, NR, R, SWF_SWAP>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case SWF_RX: dense_reg<V, NR, R, SWF_RX>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
-                    case SWF_Y: dense_reg<V, NR, R, SWF_Y>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
-                    default: dense_reg<V, NR, R, SWF_GENERAL>(op.reg, v, lidx, op.cl, m0, m1, m2, m3); break;
-                    }
+                    if (tc) dense_form_dispatch<V, NR, R, true>(op.aux, op.reg, v, cr, tok, m0, m1, m2, m3);
+                    else dense_form_dispatch<V, NR, R, false>(op.aux, op.reg, v, cr, tok, m0, m1, m2, m3);
                 } else if (op.type == SW_PHASE) {
                     const V ph1 = mkc<V>(Rl(op.m[0]), Rl(op.m[1]));
-                    const uint32_t cl = op.cl;
-                    switch (op.aux) {
-                    case SWP_I:
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(-v[k].y, v[k].x);
-                        break;
-                    case SWP_MINUS_I:
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(v[k].y, -v[k].x);
-                        break;
-                    case SWP_NEG:
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & cl) == cl) v[k] = mkc<V>(-v[k].x, -v[k].y);
-                        break;
-                    default:
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & cl) == cl) v[k] = cmul(ph1, v[k]);
-                        break;
-                    }
+                    if (tc) phase_op<V, NR, true>(op.aux, v, cr, tok, ph1);
+                    else phase_op<V, NR, false>(op.aux, v, cr, tok, ph1);
                 } else {  // SW_DIAG1
                     const V d0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), d1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
-                    if (op.dlocal >= 0) {
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & op.cl) == op.cl) v[k] = cmul(((lidx[k] >> op.dlocal) & 1) ? d1 : d0, v[k]);
-                    } else {
-                        const V d = ((base >> op.dglobal) & 1) ? d1 : d0;
-#pragma unroll
-                        for (int k = 0; k < NR; ++k)
-                            if ((lidx[k] & op.cl) == op.cl) v[k] = cmul(d, v[k]);
-                    }
+                    if (tc) diag_op<V, NR, true>(op, v, cr, tok, base, tid, d0, d1);
+                    else diag_op<V, NR, false>(op, v, cr, tok, base, tid, d0, d1);
                 }
             }
 #pragma unroll
-            for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(lidx[k]))] = v[k];
+            for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(tpart | rp[k]))] = v[k];
             __syncthreads();
         }
 
@@ -322,7 +358,7 @@ k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_
 }
 
 template <typename V, int M, int R, int SW>
-static void launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
+static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
                          const SwOp* dops, const float* dblocks, size_t cb, size_t ce)
 {
     const int n = int(st.nr_bits());
@@ -339,13 +375,139 @@ static void launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase
     sa.nphases = nphases;
     const uint64_t dim = uint64_t(1) << n;
     const unsigned gx = unsigned(std::min<uint64_t>(sa.ntiles, 2048));
+    int first = -1;
     for (size_t c0 = cb; c0 < ce; c0 += 65535) {
         const unsigned ny = unsigned(std::min<size_t>(65535, ce - c0));
         const int ps = st.prof_begin();
+        if (first < 0) first = ps;
         hipLaunchKernelGGL((k_sweep<V, M, R, SW>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
                            static_cast<V*>(st.col_ptr(c0)), dim, sa);
         SW_HIP_OK(hipGetLastError());
         st.prof_end(ps, sizeof(V) == 16 ? "k_sweep_c128" : "k_sweep_c64", 2.0 * double(dim) * sizeof(V) * ny);
+    }
+    return first;
+}
+
+// One sweep's program: phases, ops and its own block table (block phases
+// index it through op_begin until the table is merged into the flush's).
+struct SweepProgram {
+    std::vector<SwPhase> phases;
+    std::vector<SwOp> ops;
+    std::vector<float> blocks;
+    int nblocks = 0;
+};
+
+// kind 0: per-gate register phases; 3 or 4: dense blocks of that width
+static SweepProgram build_candidate(const std::vector<PendingGate>& q, const PlanStep& step, const TileGeom& g,
+                                    int kind)
+{
+    SweepProgram p;
+    if (kind == 0) build_program(q, step, g.M, g.R, p.phases, p.ops);
+    else build_fused_program(q, step, g.M, g.R, kind, p.phases, p.ops, p.blocks, &p.nblocks);
+    return p;
+}
+
+// Wall-time estimate of a gate applied on its own: one read and one write
+// of the columns at the measured streaming rate.
+static double single_gate_us(double state_bytes) { return 2.0 * state_bytes / 4.9e12 * 1e6; }
+
+// Pick the steps and sweep programs of a flush.  Plain (complex128, or
+// complex64 without fusion): per-gate register phases on the greedy tiles.
+// Fused complex64: greedy fixed-width blocks (PLAN_GREEDY), or the cheaper of
+// two tilings -- the per-gate tiling with register programs, and the fusion
+// tiling with the cheapest of {register phases, 3-qubit blocks, 4-qubit
+// blocks} per sweep -- as ranked by the learned cost model (PLAN_LEARNED), or
+// a seeded random choice among the same candidates (PLAN_EXPLORE).
+static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queue, int fusion_k, double state_bytes,
+                        double tiles, std::vector<PlanStep>& steps, std::vector<SweepProgram>& progs)
+{
+    const int n = int(st.nr_bits());
+    const TileGeom g = st.geometry();
+    const int prec = int(st.precision());
+    auto build_all = [&](const std::vector<PlanStep>& ss, int kind) {
+        std::vector<SweepProgram> out(ss.size());
+        for (size_t s = 0; s < ss.size(); ++s)
+            if (ss[s].sweep) out[s] = build_candidate(queue, ss[s], g, kind);
+        return out;
+    };
+    const bool fused = st.precision() == PREC_C64 && fusion_k > 0;
+    if (!fused) {
+        steps = plan_gates(queue, n, g, 0);
+        progs = build_all(steps, 0);
+        return;
+    }
+    // blocks of 3 or 4 local qubits: the MFMA stage is instantiated for those
+    const int Kg = std::min(4, std::max(3, fusion_k));
+    CostModel* cm = st.cost_model();
+    const int mode = st.plan_mode();
+    if (mode == PLAN_GREEDY || (mode == PLAN_LEARNED && (!cm || !cm->trained()))) {
+        steps = plan_gates(queue, n, g, Kg);
+        progs = build_all(steps, Kg);
+        return;
+    }
+    // candidates: tiling A (per-gate) with register programs; tiling B
+    // (fusion) with three programs per sweep
+    std::vector<PlanStep> sa = plan_gates(queue, n, g, 0), sb = plan_gates(queue, n, g, Kg);
+    std::vector<SweepProgram> pa = build_all(sa, 0);
+    const int kinds[3] = {0, 3, 4};
+    std::vector<std::array<SweepProgram, 3>> pb(sb.size());
+    for (size_t s = 0; s < sb.size(); ++s)
+        if (sb[s].sweep)
+            for (int c = 0; c < 3; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c]);
+    bool use_a = false;
+    std::vector<int> pick(sb.size(), 0);
+    if (mode == PLAN_EXPLORE) {
+        Xoshiro256& rng = st.plan_rng();
+        use_a = (rng.next() >> 63) != 0;
+        for (size_t s = 0; s < sb.size(); ++s) pick[s] = int(rng.next() % 3);
+    } else {
+        std::vector<float> feat;
+        std::vector<std::pair<int, int>> who;   // (step of A or -1-step of B, candidate)
+        float f[kSweepFeatures];
+        for (size_t s = 0; s < sa.size(); ++s)
+            if (sa[s].sweep) {
+                sweep_features(pa[s].phases, pa[s].ops, sa[s], prec, g, f);
+                feat.insert(feat.end(), f, f + kSweepFeatures);
+                who.emplace_back(int(s), 0);
+            }
+        for (size_t s = 0; s < sb.size(); ++s)
+            if (sb[s].sweep)
+                for (int c = 0; c < 3; ++c) {
+                    sweep_features(pb[s][c].phases, pb[s][c].ops, sb[s], prec, g, f);
+                    feat.insert(feat.end(), f, f + kSweepFeatures);
+                    who.emplace_back(-1 - int(s), c);
+                }
+        std::vector<float> y(who.size());
+        cm->predict(feat.data(), who.size(), y.data());
+        const double us_per_unit = tiles / 1024.0;   // the model predicts log(us per 1024 tiles)
+        double cost_a = 0.0, cost_b = 0.0;
+        std::vector<double> best(sb.size(), 1e300);
+        for (size_t i = 0; i < who.size(); ++i) {
+            const double us = std::exp(double(y[i])) * us_per_unit;
+            if (who[i].first >= 0) {
+                cost_a += us;
+            } else {
+                const size_t s = size_t(-1 - who[i].first);
+                if (us < best[s]) {
+                    best[s] = us;
+                    pick[s] = who[i].second;
+                }
+            }
+        }
+        for (size_t s = 0; s < sa.size(); ++s)
+            if (!sa[s].sweep) cost_a += single_gate_us(state_bytes) * double(sa[s].gates.size());
+        for (size_t s = 0; s < sb.size(); ++s)
+            cost_b += sb[s].sweep ? best[s] : single_gate_us(state_bytes) * double(sb[s].gates.size());
+        use_a = cost_a < cost_b;
+    }
+    if (use_a) {
+        steps = std::move(sa);
+        progs = std::move(pa);
+    } else {
+        steps = std::move(sb);
+        progs.resize(steps.size());
+        for (size_t s = 0; s < steps.size(); ++s)
+            if (steps[s].sweep) progs[s] = std::move(pb[s][pick[s]]);
     }
 }
 
@@ -358,42 +520,46 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         for (const PendingGate& pg : queue) st.apply_op(pg.op, col_begin, col_end);
         return;
     }
-    // dense-block fusion runs on the complex64 state (FP32 MFMA)
-    // blocks of 3 or 4 local qubits: the MFMA stage is instantiated for those
-    const int K = (st.precision() == PREC_C64 && fusion_k > 0) ? std::min(4, std::max(3, fusion_k)) : 0;
-    const std::vector<PlanStep> steps = plan_gates(queue, n, g, K);
+    const double S = double(uint64_t(1) << n) * double(st.elem_bytes()) * double(col_end - col_begin);
+    const double tiles = double(col_end - col_begin) * double(uint64_t(1) << std::max(0, n - g.M));
+    std::vector<PlanStep> steps;
+    std::vector<SweepProgram> progs;
+    choose_plan(st, queue, fusion_k, S, tiles, steps, progs);
+
     // all sweep programs of this flush in one upload: [phases][ops] per sweep, then the block table
-    std::vector<std::vector<SwPhase>> phases(steps.size());
-    std::vector<std::vector<SwOp>> ops(steps.size());
-    std::vector<float> blocks;
-    std::vector<size_t> poff(steps.size()), ooff(steps.size());
-    size_t total = 0;
+    std::vector<size_t> poff(steps.size()), ooff(steps.size()), boff_s(steps.size());
+    size_t total = 0, nblk_floats = 0;
     int nblocks = 0;
     for (size_t s = 0; s < steps.size(); ++s) {
         if (!steps[s].sweep) continue;
-        if (K) build_fused_program(queue, steps[s], g.M, g.R, K, phases[s], ops[s], blocks, &nblocks);
-        else build_program(queue, steps[s], g.M, g.R, phases[s], ops[s]);
         poff[s] = total;
-        total += (phases[s].size() * sizeof(SwPhase) + 255) / 256 * 256;
+        total += (progs[s].phases.size() * sizeof(SwPhase) + 255) / 256 * 256;
         ooff[s] = total;
-        total += (ops[s].size() * sizeof(SwOp) + 255) / 256 * 256;
+        total += (progs[s].ops.size() * sizeof(SwOp) + 255) / 256 * 256;
+        boff_s[s] = nblk_floats;
+        nblk_floats += progs[s].blocks.size();
+        nblocks += progs[s].nblocks;
     }
     const size_t boff = total;
-    total += blocks.size() * sizeof(float);
+    total += nblk_floats * sizeof(float);
     char* dprog = nullptr;
     if (total) {
         std::vector<char> host(total);
         for (size_t s = 0; s < steps.size(); ++s) {
             if (!steps[s].sweep) continue;
-            std::memcpy(host.data() + poff[s], phases[s].data(), phases[s].size() * sizeof(SwPhase));
-            std::memcpy(host.data() + ooff[s], ops[s].data(), ops[s].size() * sizeof(SwOp));
+            // block phases address the merged table
+            for (SwPhase& ph : progs[s].phases)
+                if (ph.kind == 1) ph.op_begin += uint32_t(boff_s[s]);
+            std::memcpy(host.data() + poff[s], progs[s].phases.data(), progs[s].phases.size() * sizeof(SwPhase));
+            std::memcpy(host.data() + ooff[s], progs[s].ops.data(), progs[s].ops.size() * sizeof(SwOp));
+            if (!progs[s].blocks.empty())
+                std::memcpy(host.data() + boff + boff_s[s] * sizeof(float), progs[s].blocks.data(),
+                            progs[s].blocks.size() * sizeof(float));
         }
-        if (!blocks.empty()) std::memcpy(host.data() + boff, blocks.data(), blocks.size() * sizeof(float));
         dprog = static_cast<char*>(st.scratch(GpuVectorState::SLOT_PROGRAM, total));
         SW_HIP_OK(hipMemcpy(dprog, host.data(), total, hipMemcpyHostToDevice));
     }
     const float* dblocks = reinterpret_cast<const float*>(dprog + boff);
-    const double S = double(uint64_t(1) << n) * double(st.elem_bytes()) * double(col_end - col_begin);
     for (size_t s = 0; s < steps.size(); ++s) {
         const PlanStep& step = steps[s];
         if (!step.sweep) {
@@ -402,20 +568,26 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         }
         const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
         const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
-        const int np = int(phases[s].size());
-        // the compiled tile shapes (M local bits, R register bits): see tile_shapes()
+        const int np = int(progs[s].phases.size());
+        int slot;
+        // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
         if (st.precision() == PREC_C128) {
-            if (g.M == 12 && g.R == 3) launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
-            else launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
         } else {
-            if (g.M == 13) launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
-            else launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+        }
+        if (st.recording() && col_end - col_begin <= 65535) {
+            float f[kSweepFeatures];
+            sweep_features(progs[s].phases, progs[s].ops, step, int(st.precision()), g, f);
+            st.add_sample(f, tiles, slot);
         }
         stats.launches += 1;
         stats.sweeps += 1;
         stats.gates += step.gates.size();
         stats.bytes += 2.0 * S;
-        stats.phases += phases[s].size();
+        stats.phases += progs[s].phases.size();
     }
     stats.blocks += uint64_t(nblocks);
 }

@@ -195,3 +195,25 @@ class GpuVectorState:
 
     def apply_named(self, name, bits, params=()):
         self.apply_gate(gate_matrix(name, params), bits, name)
+
+    # ---- learned planner hooks ----
+    PLAN_GREEDY, PLAN_LEARNED, PLAN_EXPLORE = 0, 1, 2
+
+    def set_cost_model(self, cm, mode=1, seed=1):
+        """Planner of fused complex64 flushes: mode 0 greedy fixed-width,
+        1 ranked by `cm` (greedy while untrained), 2 seeded random choice."""
+        check(lib().q1t_gvs_set_cost_model(self.h, cm.h if cm is not None else None, int(mode), int(seed)))
+
+    def set_recording(self, on=True):
+        check(lib().q1t_gvs_set_recording(self.h, 1 if on else 0))
+
+    def take_samples(self, cap=1 << 20):
+        """Recorded sweeps: (features n x 16 float32, tiles n, ms n)."""
+        feat = np.zeros((cap, 16), dtype=np.float32)
+        tiles = np.zeros(cap, dtype=np.float64)
+        ms = np.zeros(cap, dtype=np.float64)
+        n = ctypes.c_size_t(0)
+        check(lib().q1t_gvs_take_samples(self.h, feat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _dptr(tiles),
+                                         _dptr(ms), cap, ctypes.byref(n)))
+        k = n.value
+        return feat[:k].copy(), tiles[:k].copy(), ms[:k].copy()

@@ -1,0 +1,149 @@
+"""The learned sweep cost model on the GPU: its HIP forward, gradient and
+Adam kernels against the FP64 oracle (oracle/mlp_ref.py), training on a
+known function, and the planner it drives -- recording its own sweep
+timings, then choosing programs that leave the amplitudes as the plain
+complex128 run has them (to complex64 precision) and whose wall time is
+reported against fixed-width greedy fusion."""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as M
+from q1tsim_amd import costmodel, workloads
+from q1tsim_amd.costmodel import CostModel
+from q1tsim_amd.gvs import GpuVectorState, compare, gate_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def random_model(seed):
+    rng = np.random.default_rng(seed)
+    cm = CostModel(seed)
+    p = cm.params.astype(np.float64)
+    p[M.B1:M.W2] = rng.uniform(-0.3, 0.3, M.H)
+    p[M.B2:M.W3] = rng.uniform(-0.3, 0.3, M.H)
+    p[M.B3] = 0.25
+    cm.params = p
+    mu = rng.normal(size=16).astype(np.float32)
+    sg = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    cm.set_norm(mu, sg)
+    x = (rng.normal(size=(300, 16)) * 2 + 1).astype(np.float32)   # 300 rows: ragged last workgroup
+    t = rng.normal(size=300).astype(np.float32)
+    return cm, cm.params.astype(np.float64), mu.astype(np.float64), sg.astype(np.float64), x, t
+
+
+def test_forward_matches_oracle():
+    cm, p, mu, sg, x, _ = random_model(1)
+    y = cm.predict(x)
+    yr, _ = M.forward(p, x.astype(np.float64), mu, sg)
+    assert np.max(np.abs(y - yr)) <= 1e-5 * (1 + np.max(np.abs(yr)))
+
+
+def test_gradient_matches_oracle():
+    cm, p, mu, sg, x, t = random_model(2)
+    loss, g = cm.loss_grad(x, t)
+    lr_, gr = M.grad(p, x.astype(np.float64), t.astype(np.float64), mu, sg)
+    assert loss == pytest.approx(lr_, rel=1e-5)
+    err = np.abs(g - gr)
+    # FP32 kernels against FP64: each block of the layout to 1e-4 of its scale
+    for a, b in ((M.W1, M.B1), (M.B1, M.W2), (M.W2, M.B2), (M.B2, M.W3), (M.W3, M.B3), (M.B3, M.NPARAMS)):
+        assert err[a:b].max() <= 1e-4 * np.abs(gr[a:b]).max() + 1e-7, (a, err[a:b].max(), np.abs(gr[a:b]).max())
+
+
+def test_adam_steps_match_oracle():
+    cm, p, mu, sg, x, t = random_model(3)
+    m = np.zeros(M.NPARAMS)
+    v = np.zeros(M.NPARAMS)
+    for step in (1, 2):
+        _, g = M.grad(p, x.astype(np.float64), t.astype(np.float64), mu, sg)
+        p, m, v = M.adam_step(p, m, v, g, step, lr=1e-3)
+        cm.adam_step(x, t, lr=1e-3)
+        got = cm.params.astype(np.float64)
+        # an update is ~lr per step; FP32 gradients leave it within 1e-3 of lr
+        assert np.max(np.abs(got - p)) <= 1e-6 + 2e-6 * step, (step, np.max(np.abs(got - p)))
+
+
+def test_training_fits_known_function():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-2, 2, size=(4096, 16)).astype(np.float32)
+    t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
+    cm = CostModel(7)
+    cm.set_norm(np.zeros(16), np.ones(16))
+    l0, _ = cm.loss_grad(x, t)
+    loss = cm.train(x, t, epochs=150, batch=256, lr=3e-3, seed=1)
+    assert loss < 0.05 * l0, (l0, loss)
+
+
+N_PLAN = 20
+
+
+def c128_reference(ops, n):
+    s = GpuVectorState(n, 1, "c128")
+    for g, b, p in ops:
+        s.apply_gate(gate_matrix(g, p), b, g)
+    s.flush()
+    return s
+
+
+def run_c64(ops, n, cm=None, mode=0, fk=4):
+    s = GpuVectorState(n, 1, "c64")
+    s.set_plan(fk, True)
+    s.set_cost_model(cm, mode, 11)
+    for g, b, p in ops:
+        s.apply_gate(gate_matrix(g, p), b, g)
+    s.flush()
+    return s
+
+
+@pytest.fixture(scope="module")
+def trained():
+    circs = [workloads.random_pairs(N_PLAN, 12, seed) for seed in range(6)]
+    circs += [workloads.random_layered(N_PLAN, 8, 100 + seed) for seed in range(4)]
+    feat, tiles, ms = costmodel.record(N_PLAN, circs, fusion_k=4, seed=3, reps=2)
+    assert len(feat) > 50 and np.all(ms > 0) and np.all(tiles > 0)
+    cm = CostModel(5)
+    loss = costmodel.fit(cm, feat, tiles, ms, epochs=80)
+    assert np.isfinite(loss)
+    return cm, loss
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_planner_modes_keep_amplitudes(trained, mode):
+    cm, _ = trained
+    ops = workloads.random_pairs(N_PLAN, 16, 99)
+    ref = c128_reference(ops, N_PLAN)
+    s = run_c64(ops, N_PLAN, cm, mode)
+    infid, maxabs = compare(s, ref)
+    assert infid <= 1e-6 and maxabs <= 1e-5, (mode, infid, maxabs)
+
+
+def test_learned_plan_vs_greedy_walltime(trained):
+    """Wall time of the learned plan against greedy fixed-width fusion on
+    held-out circuits (median of 3 flushes each); written to gpurun_out."""
+    cm, loss = trained
+    n = 24
+    ops = workloads.random_pairs(n, 40, 12345)
+    res = {}
+    for mode in (0, 1):
+        s = GpuVectorState(n, 1, "c64")
+        s.set_plan(4, True)
+        s.set_cost_model(cm, mode, 1)
+        ts = []
+        for _ in range(3):
+            for g, b, p in ops:
+                s.apply_gate(gate_matrix(g, p), b, g)
+            t0 = time.perf_counter()
+            s.flush()
+            ts.append(time.perf_counter() - t0)
+        res[mode] = (float(np.median(ts)), s.stats())
+        s.close()
+    greedy, learned = res[0][0], res[1][0]
+    os.makedirs("gpurun_out", exist_ok=True)
+    with open("gpurun_out/learned_vs_greedy_n24.json", "w") as f:
+        json.dump({"n": n, "train_mse": loss, "greedy_s": greedy, "learned_s": learned,
+                   "greedy_stats": res[0][1], "learned_stats": res[1][1]}, f)
+    print("learned/greedy wall time: %.4f / %.4f s" % (learned, greedy))
+    assert learned <= 1.25 * greedy
