@@ -98,6 +98,89 @@ def cpu_baseline(ops_full, n_full, budget_s=20.0):
             "circuit_seconds_est": round(est_circuit_s, 2)}
 
 
+def batch_extra(count, seed=2024):
+    """Config: `count` independent circuits, qubits uniform in [10,22], depth
+    in [10,40], 1-3 mid-circuit Z measurements each followed by a classically
+    conditioned X or RZ, measure_all at the end, nr_runs = 64 branch columns;
+    all executed by one Batch.  Reports circuits/s per precision."""
+    from q1tsim_amd import workloads
+    from q1tsim_amd.q1tsim import Batch, Circuit
+    t0 = time.perf_counter()
+    wl = workloads.batch_workload(count, seed)
+    gen_s = time.perf_counter() - t0
+    out = {"circuits": count, "nr_runs": 64, "qubits": [10, 22], "depth": [10, 40], "measurements": [1, 3],
+           "program_ops": int(sum(len(p) for _, _, p, _ in wl)), "generate_s": round(gen_s, 2)}
+    for prec in ("c64", "c128"):
+        circs = []
+        for n, nc, prog, cs in wl:
+            c = Circuit(n, nc)
+            workloads.build_program(c, prog)
+            c.seed(cs)
+            circs.append(c)
+        b = Batch(prec, 0, 0)
+        for c in circs:
+            b.add(c)
+        t0 = time.perf_counter()
+        tried = b.autotune(64, sample=min(128, count))
+        tune_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.execute(64)
+        dt = time.perf_counter() - t0
+        st = b.stats()
+        out[prec] = {"seconds": round(dt, 3), "circuits_per_s": round(count / dt, 1), "waves": st["waves"],
+                     "rounds": st["rounds"], "sync_ops": st["sync_ops"], "gate_launches": st["launches"],
+                     "gates": st["gates"], "sweeps": st["sweeps"],
+                     "autotune": {"seconds": round(tune_s, 3), "chosen_M_R_L_fusion": list(b.config()),
+                                  "tried": [[t[0], t[1], t[2], t[3], round(t[4], 4)] for t in tried]}}
+        del b, circs
+    return out
+
+
+def learned_extra(n=32, depth=40, record_n=26, record_circuits=60, seed=5):
+    """Config: n-qubit depth-40 circuits with 25% CX on random pairs, complex64
+    with fused blocks.  The cost model is trained on sweeps the simulator
+    records itself (seeded exploration of the candidate programs at
+    `record_n` qubits), then the learned plan and greedy fixed-width fusion
+    run the held-out circuit; both are compared with the complex128 run."""
+    from q1tsim_amd import costmodel, workloads
+    from q1tsim_amd.costmodel import CostModel
+    from q1tsim_amd.gvs import GpuVectorState, compare
+    t0 = time.perf_counter()
+    circs = [workloads.random_pairs(record_n, depth, 1000 + k) for k in range(record_circuits)]
+    feat, tiles, ms = costmodel.record(record_n, circs, fusion_k=4, seed=seed)
+    rec_s = time.perf_counter() - t0
+    cm = CostModel(seed)
+    t0 = time.perf_counter()
+    mse = costmodel.fit(cm, feat, tiles, ms, epochs=40, batch=256)
+    train_s = time.perf_counter() - t0
+    blocks = int(feat[:, 1].sum() + feat[:, 2].sum())
+    ops = workloads.random_pairs(n, depth, seed)
+    ref = GpuVectorState(n, 1, "c128")
+    ref.set_plan(0, True)
+    t0 = time.perf_counter()
+    apply_circuit(ref, ops)
+    c128_s = time.perf_counter() - t0
+    res = {"n_qubits": n, "depth": depth, "gates": len(ops), "recorded_sweeps": int(len(feat)),
+           "recorded_blocks": blocks, "record_n": record_n, "record_s": round(rec_s, 2),
+           "train_s": round(train_s, 2), "train_mse_log_time": round(mse, 4), "c128_s": round(c128_s, 3)}
+    for mode, name in ((0, "greedy"), (1, "learned")):
+        s = GpuVectorState(n, 1, "c64")
+        s.set_plan(4, True)
+        s.set_cost_model(cm, mode, seed)
+        apply_circuit(s, ops)   # warm-up (and planning caches)
+        s.reset_all()
+        t0 = time.perf_counter()
+        apply_circuit(s, ops)
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        infid, maxabs = compare(ref, s)
+        res[name] = {"seconds": round(dt, 4), "sweeps": st["sweeps"], "blocks": st["blocks"],
+                     "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs}
+        del s
+    res["learned_gain"] = round(res["greedy"]["seconds"] / res["learned"]["seconds"], 4)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +190,8 @@ def main():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--fusion-k", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the QFT and sampling lines")
+    ap.add_argument("--no-extra", action="store_true", help="skip the QFT, sampling, batch and learned lines")
+    ap.add_argument("--batch-circuits", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU sharding is not part of this build (run with --gpus 1)",
@@ -150,6 +234,15 @@ def main():
         t0 = time.perf_counter()
         plain.measure_all_into(list(range(n)), res, Xoshiro(1), collapse=False)
         extra["sample65536_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
+        del plain
+        try:
+            extra["batch"] = batch_extra(args.batch_circuits)
+        except Exception as e:   # reported, not hidden: the headline line still prints
+            extra["batch"] = {"error": repr(e)}
+        try:
+            extra["learned_planner"] = learned_extra()
+        except Exception as e:
+            extra["learned_planner"] = {"error": repr(e)}
 
     value = bytes_plain / (ms_plain * 1e-3) / 1e9
     unf = workloads.unfused_bytes(ops, S128)

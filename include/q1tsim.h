@@ -108,6 +108,19 @@ int q1t_batch_add(q1t_batch* b, circuit_t* c);
 /* keep_states != 0: leave each end state on the device (circuit reads work) */
 int q1t_batch_execute(q1t_batch* b, size_t nr_shots, int keep_states);
 int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out);
+/* Autotuner: executes the first `sample` circuits under every compiled tile
+ * shape (and, for complex64, fusion widths 0 and 4), keeps the fastest, and
+ * restores each sampled circuit's random stream.  Writes up to cap timed
+ * configurations; *count = configurations timed. */
+typedef struct q1t_batch_config {
+    int M, R, L;            /* tile bits, register bits, low bits per tile */
+    int fusion_k;           /* 0: per-gate arithmetic; 3..4: dense blocks */
+    double seconds;         /* wall time of the sample (0 for get_config) */
+} q1t_batch_config;
+int q1t_batch_autotune(q1t_batch* b, size_t nr_shots, size_t sample, q1t_batch_config* out, size_t cap,
+                       size_t* count);
+int q1t_batch_set_config(q1t_batch* b, const q1t_batch_config* cfg);
+int q1t_batch_get_config(const q1t_batch* b, q1t_batch_config* out);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,11 @@ class BatchStats(ctypes.Structure):
                 ("sync_ops", ctypes.c_uint64), ("seconds", ctypes.c_double)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int), ("R", ctypes.c_int), ("L", ctypes.c_int), ("fusion_k", ctypes.c_int),
+                ("seconds", ctypes.c_double)]
+
+
 class Rng(ctypes.Structure):
     _fields_ = [("ctx", ctypes.c_void_p), ("next_u64", ctypes.CFUNCTYPE(ctypes.c_uint64, ctypes.c_void_p))]
 
@@ -102,6 +107,9 @@ SIGNATURES = {
     "q1t_batch_add": (_i32, [_vp, _vp]),
     "q1t_batch_execute": (_i32, [_vp, _sz, _i32]),
     "q1t_batch_stats_get": (_i32, [_vp, ctypes.POINTER(BatchStats)]),
+    "q1t_batch_autotune": (_i32, [_vp, _sz, _sz, ctypes.POINTER(BatchConfig), _sz, _ps]),
+    "q1t_batch_set_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
+    "q1t_batch_get_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
     "q1t_gvs_new": (_i32, [_sz, _sz, _i32, ctypes.POINTER(_vp)]),
     "q1t_gvs_free": (None, [_vp]),
     "q1t_gvs_from_qubit_coefs": (_i32, [_vp, _pd, _sz]),

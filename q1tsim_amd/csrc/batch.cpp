@@ -7,8 +7,53 @@
 namespace q1t {
 
 Batch::Batch(Precision prec, int fusion_k, size_t mem_budget)
-    : prec_(prec), fusion_k_(fusion_k), mem_budget_(mem_budget)
+    : prec_(prec), fusion_k_(fusion_k), geom_(tile_geometry(prec)), mem_budget_(mem_budget)
 {
+}
+
+void Batch::set_config(const TileGeom& g, int fusion_k)
+{
+    if (!tile_shape_supported(prec_, g.M, g.R) || g.L < 1 || g.L > g.M)
+        throw Error(ERR_INVALID_ARGUMENT, "Unsupported tile geometry for this precision");
+    if (fusion_k < 0 || fusion_k > 6) throw Error(ERR_INVALID_ARGUMENT, "fusion_k must be 0..6");
+    geom_ = g;
+    fusion_k_ = fusion_k;
+    for (Circuit* c : circuits_) c->set_backend(prec_, fusion_k_, true);
+}
+
+std::vector<BatchConfig> Batch::autotune(size_t nr_shots, size_t sample)
+{
+    std::vector<BatchConfig> cand;
+    if (prec_ == PREC_C128) {
+        cand.push_back({TileGeom{12, 4, 4}, 0, 0.0});
+        cand.push_back({TileGeom{12, 3, 4}, 0, 0.0});
+    } else {
+        for (int k : {0, 4}) {
+            cand.push_back({TileGeom{12, 4, 4}, k, 0.0});
+            cand.push_back({TileGeom{13, 4, 5}, k, 0.0});
+        }
+    }
+    sample = std::min(sample, circuits_.size());
+    if (!sample) return cand;
+    std::vector<Circuit*> part(circuits_.begin(), circuits_.begin() + std::ptrdiff_t(sample));
+    std::vector<Xoshiro256> saved;
+    for (Circuit* c : part) saved.push_back(c->rng_state());
+    size_t best = 0;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        geom_ = cand[i].geom;
+        fusion_k_ = cand[i].fusion_k;
+        for (size_t k = 0; k < part.size(); ++k) {
+            part[k]->set_backend(prec_, fusion_k_, true);
+            part[k]->set_rng_state(saved[k]);
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        run(part, nr_shots, false);
+        cand[i].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (cand[i].seconds < cand[best].seconds) best = i;
+    }
+    for (size_t k = 0; k < part.size(); ++k) part[k]->set_rng_state(saved[k]);
+    set_config(cand[best].geom, cand[best].fusion_k);
+    return cand;
 }
 
 void Batch::add(Circuit* c)
@@ -47,6 +92,12 @@ void Batch::execute(size_t nr_shots, bool keep_states)
 {
     const auto t0 = std::chrono::steady_clock::now();
     stats_ = BatchStats{};
+    run(circuits_, nr_shots, keep_states);
+    stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_states)
+{
     if (!stream_) stream_ = std::make_shared<SharedStream>();
     size_t budget = mem_budget_;
     if (!budget) {
@@ -55,19 +106,22 @@ void Batch::execute(size_t nr_shots, bool keep_states)
         budget = free_b / 4 * 3;
     }
     size_t next = 0;
-    while (next < circuits_.size()) {
+    while (next < list.size()) {
         // one wave: as many circuits, in order, as the budget holds (at least one)
         std::vector<Circuit*> wave;
         size_t used = 0;
-        while (next < circuits_.size()) {
-            const size_t b = resident_bytes(*circuits_[next], nr_shots);
+        while (next < list.size()) {
+            const size_t b = resident_bytes(*list[next], nr_shots);
             if (!wave.empty() && used + b > budget) break;
-            wave.push_back(circuits_[next]);
+            wave.push_back(list[next]);
             used += b;
             ++next;
         }
         stats_.waves += 1;
-        for (Circuit* c : wave) c->start(nr_shots, stream_);
+        for (Circuit* c : wave) {
+            c->start(nr_shots, stream_);
+            c->state()->set_geometry(geom_);
+        }
         std::vector<Circuit*> active = wave;
         std::vector<GpuVectorState*> states;
         std::vector<char> more;
@@ -94,7 +148,6 @@ void Batch::execute(size_t nr_shots, bool keep_states)
             active.swap(still);
         }
     }
-    stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // namespace q1t

@@ -20,6 +20,13 @@ struct BatchStats {
     double seconds = 0.0;      // wall time of execute (host clock)
 };
 
+// One configuration the autotuner timed.
+struct BatchConfig {
+    TileGeom geom;
+    int fusion_k;
+    double seconds;   // wall time of the sample under this configuration
+};
+
 class Batch {
 public:
     // `mem_budget`: device bytes the resident states of one wave may take
@@ -35,9 +42,19 @@ public:
     void execute(size_t nr_shots, bool keep_states);
     const BatchStats& stats() const { return stats_; }
 
+    // Autotuner: execute the first `sample` circuits under every compiled
+    // tile shape of the precision and, for complex64, fusion widths {0, 4};
+    // keep the fastest for execute.  Each sampled circuit's random stream is
+    // restored afterwards, so execute still gives the results of
+    // Circuit::execute.  Returns every configuration timed.
+    std::vector<BatchConfig> autotune(size_t nr_shots, size_t sample);
+    void set_config(const TileGeom& g, int fusion_k);
+    BatchConfig config() const { return BatchConfig{geom_, fusion_k_, 0.0}; }
+
 private:
     Precision prec_;
     int fusion_k_;
+    TileGeom geom_;
     size_t mem_budget_;
     std::vector<Circuit*> circuits_;
     std::shared_ptr<SharedStream> stream_;
@@ -45,6 +62,7 @@ private:
     BatchStats stats_;
 
     size_t resident_bytes(const Circuit& c, size_t nr_shots) const;
+    void run(const std::vector<Circuit*>& circuits, size_t nr_shots, bool keep_states);
 };
 
 }  // namespace q1t

@@ -388,6 +388,35 @@ extern "C" int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out)
     return 0;
 }
 
+extern "C" int q1t_batch_autotune(q1t_batch* b, size_t nr_shots, size_t sample, q1t_batch_config* out, size_t cap,
+                                  size_t* count)
+{
+    NEED(b);
+    NEED(count);
+    return guard([&] {
+        std::vector<BatchConfig> v = b->b.autotune(nr_shots, sample);
+        *count = v.size();
+        for (size_t i = 0; i < v.size() && i < cap && out; ++i)
+            out[i] = q1t_batch_config{v[i].geom.M, v[i].geom.R, v[i].geom.L, v[i].fusion_k, v[i].seconds};
+    });
+}
+
+extern "C" int q1t_batch_set_config(q1t_batch* b, const q1t_batch_config* cfg)
+{
+    NEED(b);
+    NEED(cfg);
+    return guard([&] { b->b.set_config(TileGeom{cfg->M, cfg->R, cfg->L}, cfg->fusion_k); });
+}
+
+extern "C" int q1t_batch_get_config(const q1t_batch* b, q1t_batch_config* out)
+{
+    NEED(b);
+    NEED(out);
+    const BatchConfig c = b->b.config();
+    *out = q1t_batch_config{c.geom.M, c.geom.R, c.geom.L, c.fusion_k, 0.0};
+    return 0;
+}
+
 // ------------------------------------------------- state-level ABI
 
 static std::vector<cdouble> to_cd(const double* m, size_t n)

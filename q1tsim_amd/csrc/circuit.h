@@ -73,6 +73,8 @@ public:
     // build-added
     void set_backend(Precision prec, int fusion_k, bool tiling);
     void seed(uint64_t s) { rng_.reseed(s); }
+    Xoshiro256 rng_state() const { return rng_; }
+    void set_rng_state(const Xoshiro256& r) { rng_ = r; }
     void set_initial_amplitudes(std::vector<cdouble> amps);
     void set_initial_random(uint64_t seed);
     GpuVectorState* state() { return state_.get(); }

@@ -63,13 +63,13 @@ template <typename V> __device__ __forceinline__ V sel(bool c, V a, V b)
 // FORM picks the body from the matrix structure, as the reference's per-gate
 // apply_mat_slice bodies do: general complex, real (H, RY), swap (X, CX),
 // [[a, ib], [ib, a]] (RX) and [[0, -i], [i, 0]] (Y).
-template <typename V, int NR, int J, int FORM, bool TC>
+template <typename V, int NR, int J, int FORM, bool TC, bool RC>
 __device__ __forceinline__ void dense_pairs(V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         if ((k >> J) & 1) continue;
-        if ((uint32_t(k) & cr) != cr) continue;
+        if (RC && (uint32_t(k) & cr) != cr) continue;
         const int k1 = k | (1 << J);
         const V x = v[k], y = v[k1];
         V nx, ny;
@@ -100,69 +100,97 @@ __device__ __forceinline__ void dense_pairs(V (&v)[NR], uint32_t cr, bool tok, V
     }
 }
 
-template <typename V, int NR, int R, int FORM, bool TC>
+template <typename V, int NR, int R, int FORM, bool TC, bool RC>
 __device__ __forceinline__ void dense_reg(int reg, V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
     switch (reg) {
-    case 0: dense_pairs<V, NR, 0, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 1: dense_pairs<V, NR, 1, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 2: dense_pairs<V, NR, 2, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 3: dense_pairs<V, NR, 3, FORM, TC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 0: dense_pairs<V, NR, 0, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 1: dense_pairs<V, NR, 1, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 2: dense_pairs<V, NR, 2, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 3: dense_pairs<V, NR, 3, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
     default:
-        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM, TC>(v, cr, tok, m0, m1, m2, m3);
+        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3);
         break;
     }
 }
 
-template <typename V, int NR, int R, bool TC>
-__device__ __forceinline__ void dense_form_dispatch(uint32_t form, int reg, V (&v)[NR], uint32_t cr, bool tok, V m0,
-                                                    V m1, V m2, V m3)
+// Controls by where they live: thread bits (TC, a per-lane predicate) and
+// register bits (RC, a per-slot predicate); each combination has its own
+// straight-line body, so the common uncontrolled op carries no selects.
+template <typename V, int NR, int R, int FORM>
+__device__ __forceinline__ void dense_ctl(int reg, V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, V m0, V m1, V m2,
+                                          V m3)
+{
+    if (ct) {
+        if (cr) dense_reg<V, NR, R, FORM, true, true>(reg, v, cr, tok, m0, m1, m2, m3);
+        else dense_reg<V, NR, R, FORM, true, false>(reg, v, cr, tok, m0, m1, m2, m3);
+    } else {
+        if (cr) dense_reg<V, NR, R, FORM, false, true>(reg, v, cr, tok, m0, m1, m2, m3);
+        else dense_reg<V, NR, R, FORM, false, false>(reg, v, cr, tok, m0, m1, m2, m3);
+    }
+}
+
+template <typename V, int NR, int R>
+__device__ __forceinline__ void dense_form_dispatch(uint32_t form, int reg, V (&v)[NR], uint32_t cr, uint32_t ct,
+                                                    bool tok, V m0, V m1, V m2, V m3)
 {
     switch (form) {
-    case SWF_REAL: dense_reg<V, NR, R, SWF_REAL, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
-    case SWF_SWAP: dense_reg<V, NR, R, SWF_SWAP, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
-    case SWF_RX: dense_reg<V, NR, R, SWF_RX, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
-    case SWF_Y: dense_reg<V, NR, R, SWF_Y, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
-    default: dense_reg<V, NR, R, SWF_GENERAL, TC>(reg, v, cr, tok, m0, m1, m2, m3); break;
+    case SWF_REAL: dense_ctl<V, NR, R, SWF_REAL>(reg, v, cr, ct, tok, m0, m1, m2, m3); break;
+    case SWF_SWAP: dense_ctl<V, NR, R, SWF_SWAP>(reg, v, cr, ct, tok, m0, m1, m2, m3); break;
+    case SWF_RX: dense_ctl<V, NR, R, SWF_RX>(reg, v, cr, ct, tok, m0, m1, m2, m3); break;
+    case SWF_Y: dense_ctl<V, NR, R, SWF_Y>(reg, v, cr, ct, tok, m0, m1, m2, m3); break;
+    default: dense_ctl<V, NR, R, SWF_GENERAL>(reg, v, cr, ct, tok, m0, m1, m2, m3); break;
     }
 }
 
 // Multiply the registers whose register controls `cr` are set (and, with TC,
 // whose lane passes `tok`) by the value f(k, v[k]) gives.
-template <typename V, int NR, bool TC, typename F>
-__device__ __forceinline__ void scale_regs(V (&v)[NR], uint32_t cr, bool tok, F f)
+template <typename V, int NR, bool TC, bool RC, typename F>
+__device__ __forceinline__ void scale_regs_t(V (&v)[NR], uint32_t cr, bool tok, F f)
 {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        if ((uint32_t(k) & cr) != cr) continue;
+        if (RC && (uint32_t(k) & cr) != cr) continue;
         const V nv = f(k, v[k]);
         if constexpr (TC) v[k] = sel(tok, nv, v[k]);
         else v[k] = nv;
     }
 }
 
-template <typename V, int NR, bool TC>
-__device__ __forceinline__ void phase_op(uint32_t form, V (&v)[NR], uint32_t cr, bool tok, V ph1)
+template <typename V, int NR, typename F>
+__device__ __forceinline__ void scale_regs(V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, F f)
 {
-    switch (form) {
-    case SWP_I: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(-x.y, x.x); }); break;
-    case SWP_MINUS_I: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(x.y, -x.x); }); break;
-    case SWP_NEG: scale_regs<V, NR, TC>(v, cr, tok, [](int, V x) { return mkc<V>(-x.x, -x.y); }); break;
-    default: scale_regs<V, NR, TC>(v, cr, tok, [&](int, V x) { return cmul(ph1, x); }); break;
+    if (ct) {
+        if (cr) scale_regs_t<V, NR, true, true>(v, cr, tok, f);
+        else scale_regs_t<V, NR, true, false>(v, cr, tok, f);
+    } else {
+        if (cr) scale_regs_t<V, NR, false, true>(v, cr, tok, f);
+        else scale_regs_t<V, NR, false, false>(v, cr, tok, f);
     }
 }
 
-template <typename V, int NR, bool TC>
-__device__ __forceinline__ void diag_op(const SwOp& op, V (&v)[NR], uint32_t cr, bool tok, uint64_t base, int tid,
-                                        V d0, V d1)
+template <typename V, int NR>
+__device__ __forceinline__ void phase_op(uint32_t form, V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, V ph1)
+{
+    switch (form) {
+    case SWP_I: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(-x.y, x.x); }); break;
+    case SWP_MINUS_I: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(x.y, -x.x); }); break;
+    case SWP_NEG: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(-x.x, -x.y); }); break;
+    default: scale_regs<V, NR>(v, cr, ct, tok, [&](int, V x) { return cmul(ph1, x); }); break;
+    }
+}
+
+template <typename V, int NR>
+__device__ __forceinline__ void diag_op(const SwOp& op, V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, uint64_t base,
+                                        int tid, V d0, V d1)
 {
     if (op.dreg >= 0) {
         const int dr = op.dreg;
-        scale_regs<V, NR, TC>(v, cr, tok, [&](int k, V x) { return cmul(sel(((k >> dr) & 1) != 0, d1, d0), x); });
+        scale_regs<V, NR>(v, cr, ct, tok, [&](int k, V x) { return cmul(sel(((k >> dr) & 1) != 0, d1, d0), x); });
     } else {
         const bool one = op.dtid >= 0 ? ((tid >> op.dtid) & 1) : ((base >> op.dglobal) & 1);
         const V d = sel(one, d1, d0);
-        scale_regs<V, NR, TC>(v, cr, tok, [&](int, V x) { return cmul(d, x); });
+        scale_regs<V, NR>(v, cr, ct, tok, [&](int, V x) { return cmul(d, x); });
     }
 }
 
@@ -292,22 +320,17 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
                 const SwOp& op = sa.ops[oi];
                 if ((base & op.co) != op.co) continue;
-                const uint32_t cr = op.cr;
-                const bool tc = op.ct != 0;
-                const bool tok = (uint32_t(tid) & op.ct) == op.ct;
+                const uint32_t cr = op.cr, ct = op.ct;
+                const bool tok = (uint32_t(tid) & ct) == ct;
                 if (op.type == SW_DENSE1) {
                     const V m0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), m1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
                     const V m2 = mkc<V>(Rl(op.m[4]), Rl(op.m[5])), m3 = mkc<V>(Rl(op.m[6]), Rl(op.m[7]));
-                    if (tc) dense_form_dispatch<V, NR, R, true>(op.aux, op.reg, v, cr, tok, m0, m1, m2, m3);
-                    else dense_form_dispatch<V, NR, R, false>(op.aux, op.reg, v, cr, tok, m0, m1, m2, m3);
+                    dense_form_dispatch<V, NR, R>(op.aux, op.reg, v, cr, ct, tok, m0, m1, m2, m3);
                 } else if (op.type == SW_PHASE) {
-                    const V ph1 = mkc<V>(Rl(op.m[0]), Rl(op.m[1]));
-                    if (tc) phase_op<V, NR, true>(op.aux, v, cr, tok, ph1);
-                    else phase_op<V, NR, false>(op.aux, v, cr, tok, ph1);
+                    phase_op<V, NR>(op.aux, v, cr, ct, tok, mkc<V>(Rl(op.m[0]), Rl(op.m[1])));
                 } else {  // SW_DIAG1
                     const V d0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), d1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
-                    if (tc) diag_op<V, NR, true>(op, v, cr, tok, base, tid, d0, d1);
-                    else diag_op<V, NR, false>(op, v, cr, tok, base, tid, d0, d1);
+                    diag_op<V, NR>(op, v, cr, ct, tok, base, tid, d0, d1);
                 }
             }
 #pragma unroll

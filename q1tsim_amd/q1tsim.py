@@ -346,6 +346,24 @@ class Batch(object):
     def execute(self, nr_shots, keep_states=False):
         _lib.check(self.__sim.q1t_batch_execute(self.__ptr, int(nr_shots), 1 if keep_states else 0))
 
+    def autotune(self, nr_shots, sample=64):
+        """Time the first `sample` circuits under each compiled tile shape
+        (and complex64 fusion width); keep the fastest.  Returns
+        [(M, R, L, fusion_k, seconds)] for every configuration timed."""
+        arr = (_lib.BatchConfig * 16)()
+        n = ctypes.c_size_t(0)
+        _lib.check(self.__sim.q1t_batch_autotune(self.__ptr, int(nr_shots), int(sample), arr, 16, ctypes.byref(n)))
+        return [(arr[i].M, arr[i].R, arr[i].L, arr[i].fusion_k, arr[i].seconds) for i in range(min(n.value, 16))]
+
+    def config(self):
+        c = _lib.BatchConfig()
+        _lib.check(self.__sim.q1t_batch_get_config(self.__ptr, ctypes.byref(c)))
+        return (c.M, c.R, c.L, c.fusion_k)
+
+    def set_config(self, M, R, L, fusion_k):
+        c = _lib.BatchConfig(M, R, L, fusion_k, 0.0)
+        _lib.check(self.__sim.q1t_batch_set_config(self.__ptr, ctypes.byref(c)))
+
     def stats(self):
         s = _lib.BatchStats()
         _lib.check(self.__sim.q1t_batch_stats_get(self.__ptr, ctypes.byref(s)))

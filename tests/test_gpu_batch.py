@@ -115,3 +115,22 @@ def test_batch_same_circuit_many_seeds():
         assert c.histogram() == solo.histogram()
         total += sum(c.histogram_vec())
     assert total == 32 * SHOTS
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_autotune_keeps_results(prec):
+    # the autotuner executes a sample under each configuration; afterwards
+    # every circuit still gives what it gives alone with its seed
+    batch = Batch(prec)
+    circs = [make(n, nc, prog, s, prec) for (n, nc, prog, s) in WL]
+    for c in circs:
+        batch.add(c)
+    tried = batch.autotune(SHOTS, sample=8)
+    assert len(tried) == (2 if prec == "c128" else 4) and all(t[4] > 0 for t in tried)
+    best = min(tried, key=lambda t: t[4])
+    assert batch.config() == best[:4]
+    batch.execute(SHOTS)
+    for (n, nc, prog, s), c in zip(WL, circs):
+        solo = make(n, nc, prog, s, prec, best[3])
+        solo.execute(SHOTS)
+        assert np.array_equal(np.array(c.cstate()), np.array(solo.cstate())), (n, s)
