@@ -20,9 +20,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 # Packed FP32 (v_pk_*_f32) stays off for all device code.  Sinking of
 # common code out of switch cases is off: it merges the per-register-slot
 # cases of the sweep kernels into one body with a run-time index into the
-# register array, which puts that array in scratch memory.
+# register array, which puts that array in scratch memory.  Uniform regions
+# are left unstructurized: the sweep kernels dispatch on wave-uniform op
+# fields, and structurizing those branches copies the whole register array
+# at every join.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics", "-mllvm", "-simplifycfg-sink-common=false"]
+          "-munsafe-fp-atomics", "-mllvm", "-simplifycfg-sink-common=false",
+          "-mllvm", "-structurizecfg-skip-uniform-regions"]
 
 SOURCES = ["gvs.hip", "sweep.hip", "costmodel.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp",
            "capi.cpp"]
