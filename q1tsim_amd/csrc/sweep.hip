@@ -39,7 +39,6 @@ struct SweepArgs {
     const SwOp* ops;
     const float* blocks;   // real-embedded block matrices (fused segments)
     int nphases;
-    int io;              // bit 0: the first phase reads HBM directly, bit 1: the last phase writes it
 };
 
 // LDS slot of local element l: XOR-fold higher bits into the low SW bits so
@@ -284,12 +283,9 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
 
     {
         const uint64_t base = deposit(tile, sa.tset);
-        const bool din = sa.io & 1, dout = (sa.io >> 1) & 1;
-        if (!din) {
 #pragma unroll
-            for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | g_tid | g_k[k]];
-            __syncthreads();
-        }
+        for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | g_tid | g_k[k]];
+        __syncthreads();
 
         for (int p = 0; p < sa.nphases; ++p) {
             const SwPhase& ph = sa.phases[p];
@@ -318,32 +314,9 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
                     if ((k >> b) & 1) l |= 1u << ph.rpos[b];
                 rp[k] = l;
             }
-            // direct HBM access of the first / last phase: global offsets of
-            // this lane's thread bits and of each register slot
-            const bool hbm_in = din && p == 0, hbm_out = dout && p == sa.nphases - 1;
-            uint64_t gpart = 0;
-            uint64_t grp[NR];
-            if (hbm_in || hbm_out) {
-#pragma unroll
-                for (int b = 0; b < M - R; ++b)
-                    if ((tid >> b) & 1) gpart |= uint64_t(1) << sa.tpos[ph.tbit[b]];
-#pragma unroll
-                for (int k = 0; k < NR; ++k) {
-                    uint64_t gk = 0;
-#pragma unroll
-                    for (int b = 0; b < R; ++b)
-                        if ((k >> b) & 1) gk |= uint64_t(1) << sa.tpos[ph.rpos[b]];
-                    grp[k] = base | gk;
-                }
-            }
             V v[NR];
-            if (hbm_in) {
 #pragma unroll
-                for (int k = 0; k < NR; ++k) v[k] = a[grp[k] | gpart];
-            } else {
-#pragma unroll
-                for (int k = 0; k < NR; ++k) v[k] = lds[swz<SW, M>(int(tpart | rp[k]))];
-            }
+            for (int k = 0; k < NR; ++k) v[k] = lds[swz<SW, M>(int(tpart | rp[k]))];
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
                 const SwOp& op = sa.ops[oi];
                 if ((base & op.co) != op.co) continue;
@@ -360,21 +333,14 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
                     diag_op<V, NR>(op, v, cr, ct, tok, base, tid, d0, d1);
                 }
             }
-            if (hbm_out) {
 #pragma unroll
-                for (int k = 0; k < NR; ++k) a[grp[k] | gpart] = v[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(tpart | rp[k]))] = v[k];
-                __syncthreads();
-            }
+            for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(tpart | rp[k]))] = v[k];
+            __syncthreads();
         }
 
-        if (!dout) {
 #pragma unroll
-            for (int k = 0; k < NR; ++k) a[base | g_tid | g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
-        }
-        __syncthreads();   // the next tile reuses the LDS
+        for (int k = 0; k < NR; ++k) a[base | g_tid | g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
+        __syncthreads();
     }
 }
 
@@ -416,7 +382,7 @@ k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_
 
 template <typename V, int M, int R, int SW>
 static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
-                        const SwOp* dops, const float* dblocks, int io, size_t cb, size_t ce)
+                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce)
 {
     const int n = int(st.nr_bits());
     SweepArgs sa{};
@@ -430,7 +396,6 @@ static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase*
     sa.ops = dops;
     sa.blocks = dblocks;
     sa.nphases = nphases;
-    sa.io = io;
     const uint64_t dim = uint64_t(1) << n;
     const unsigned gx = unsigned(std::min<uint64_t>(sa.ntiles, 2048));
     int first = -1;
@@ -444,17 +409,6 @@ static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase*
         st.prof_end(ps, sizeof(V) == 16 ? "k_sweep_c128" : "k_sweep_c64", 2.0 * double(dim) * sizeof(V) * ny);
     }
     return first;
-}
-
-// A register phase moves its amplitudes between HBM and registers directly
-// when none of the L lowest local bits is a register bit: every group of 2^L
-// lanes then reads or writes 2^L contiguous amplitudes.  Bit 0: the first
-// phase can, bit 1: the last phase can.
-static int direct_io(const std::vector<SwPhase>& ph, int L)
-{
-    if (ph.empty()) return 0;
-    auto ok = [&](const SwPhase& p) { return p.kind == 0 && (p.reg_mask & ((1u << L) - 1)) == 0; };
-    return (ok(ph.front()) ? 1 : 0) | (ok(ph.back()) ? 2 : 0);
 }
 
 // One sweep's program: phases, ops and its own block table (block phases
@@ -638,15 +592,14 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
         const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
         const int np = int(progs[s].phases.size());
-        const int io = direct_io(progs[s].phases, g.L);
         int slot;
         // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
         if (st.precision() == PREC_C128) {
-            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, io, col_begin, col_end);
-            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, io, col_begin, col_end);
+            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
         } else {
-            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, io, col_begin, col_end);
-            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, io, col_begin, col_end);
+            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
         }
         if (st.recording() && col_end - col_begin <= 65535) {
             float f[kSweepFeatures];
@@ -872,7 +825,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
     struct Plan {
         std::vector<PlanStep> steps;
         std::vector<size_t> poff, ooff;
-        std::vector<int> nph, io;
+        std::vector<int> nph;
         bool small = false;
         int sop_begin = 0, sop_end = 0;
         RunStats st;
@@ -911,7 +864,6 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         p.poff.assign(p.steps.size(), 0);
         p.ooff.assign(p.steps.size(), 0);
         p.nph.assign(p.steps.size(), 0);
-        p.io.assign(p.steps.size(), 0);
         int nblocks = 0;
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
@@ -928,7 +880,6 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
             host.insert(host.end(), reinterpret_cast<const char*>(ops.data()),
                         reinterpret_cast<const char*>(ops.data() + ops.size()));
             p.nph[s] = int(ph.size());
-            p.io[s] = direct_io(ph, g.L);
             p.st.sweeps += 1;
             p.st.launches += 1;
             p.st.gates += p.steps[s].gates.size();
@@ -997,7 +948,6 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
                 it.sa.ops = reinterpret_cast<const SwOp*>(dbase + p.ooff[r]);
                 it.sa.blocks = reinterpret_cast<const float*>(dbase + boff);
                 it.sa.nphases = p.nph[r];
-                it.sa.io = p.io[r];
                 std::memcpy(host.data() + items_off + wi * sizeof(SweepItem), &it, sizeof(it));
                 tiles += it.sa.ntiles;
                 ++wi;
