@@ -192,6 +192,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the QFT, sampling, batch and learned lines")
     ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--profile-workload", action="store_true",
+                    help="one pass for rocprofv3 counter runs: k_scale over the c128 state (a read+write "
+                         "of known size, for scaling FETCH_SIZE/WRITE_SIZE), then the plain circuit once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU sharding is not part of this build (run with --gpus 1)",
@@ -200,6 +203,17 @@ def main():
 
     from q1tsim_amd import workloads
     from q1tsim_amd.gvs import GpuVectorState, compare
+
+    if args.profile_workload:
+        n = args.n
+        st = GpuVectorState(n, 1, "c128")
+        st.set_plan(0, True)
+        st.set_random_normal(3)   # k_random_normal, k_norm_partial, k_scale (reads and writes 16 * 2^n bytes)
+        apply_circuit(st, workloads.random_layered(n, args.depth, 1))
+        s = st.stats()
+        print(json.dumps({"profile_workload": True, "n": n, "state_bytes": 16 << n, "sweeps": s["sweeps"],
+                          "sweep_bytes_per_launch": 2 * (16 << n)}))
+        return
 
     n = args.n
     ops = workloads.random_layered(n, args.depth, 1)
