@@ -11,6 +11,28 @@ Batch::Batch(Precision prec, int fusion_k, size_t mem_budget)
 {
 }
 
+Batch::~Batch()
+{
+    if (host_) (void)hipHostFree(host_);
+}
+
+double* Batch::pinned(size_t n)
+{
+    if (n > host_cap_) {
+        if (host_) {
+            if (stream_) (void)hipStreamSynchronize(stream_->s);
+            (void)hipHostFree(host_);
+            host_ = nullptr;
+            host_cap_ = 0;
+        }
+        const size_t cap = std::max<size_t>(n, size_t(1) << 16);
+        if (hipHostMalloc(reinterpret_cast<void**>(&host_), cap * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            throw Error(ERR_OUT_OF_MEMORY, "Pinned host allocation failed");
+        host_cap_ = cap;
+    }
+    return host_;
+}
+
 void Batch::set_config(const TileGeom& g, int fusion_k)
 {
     if (!tile_shape_supported(prec_, g.M, g.R) || g.L < 1 || g.L > g.M)
@@ -63,29 +85,59 @@ void Batch::add(Circuit* c)
     circuits_.push_back(c);
 }
 
-// Upper bound on the device bytes of a circuit's state: every collapsing
-// single-qubit measurement, reset or conditional gate may split each column
-// in two (vectorstate.rs:193-227, 237-329, 402-408); a collapsing
-// measure_all leaves up to one column per run.
+// Upper bound on the device bytes of a circuit's state.  A collapsing
+// single-qubit measurement or a reset may split every column in two
+// (vectorstate.rs:237-329, 402-408); a conditional gate splits columns only
+// when some run disagrees with the rest of its column on a control bit
+// (vectorstate.rs:193-227), which cannot happen for control bits written by
+// an earlier collapsing measurement (each column then holds one outcome); a
+// collapsing measure_all leaves basis columns that take device memory only
+// if gates follow.
 size_t Batch::resident_bytes(const Circuit& c, size_t nr_shots) const
 {
-    size_t cols = 1;
-    for (const CircuitOp& op : c.ops()) {
+    const std::vector<CircuitOp>& ops = c.ops();
+    size_t cols = 1, peak = 1;
+    std::vector<char> det(c.nr_cbits(), 0);
+    for (size_t i = 0; i < ops.size(); ++i) {
+        const CircuitOp& op = ops[i];
         switch (op.type) {
         case OpType::Measure:
+            cols = std::min(nr_shots, cols * 2);
+            det[op.cbit] = 1;
+            break;
+        case OpType::Peek:
+            det[op.cbit] = 0;
+            break;
         case OpType::Reset:
-        case OpType::CondGate:
             cols = std::min(nr_shots, cols * 2);
             break;
-        case OpType::MeasureAll:
-            cols = nr_shots;
+        case OpType::CondGate: {
+            bool all = true;
+            for (size_t b : op.control) all = all && det[b];
+            if (!all) cols = std::min(nr_shots, cols * 2);
+            break;
+        }
+        case OpType::MeasureAll: {
+            bool gates_follow = false;
+            for (size_t j = i + 1; j < ops.size(); ++j)
+                gates_follow = gates_follow || ops[j].type == OpType::Gate || ops[j].type == OpType::CondGate;
+            if (gates_follow) cols = nr_shots;
+            for (size_t b : op.cbits) det[b] = 1;
+            break;
+        }
+        case OpType::PeekAll:
+            for (size_t b : op.cbits) det[b] = 0;
+            break;
+        case OpType::ResetAll:
+            cols = 1;
             break;
         default:
             break;
         }
+        peak = std::max(peak, cols);
     }
     const size_t elem = prec_ == PREC_C128 ? 16 : 8;
-    return std::max<size_t>(cols, 1) * (size_t(1) << c.nr_qbits()) * elem;
+    return peak * (size_t(1) << c.nr_qbits()) * elem;
 }
 
 void Batch::execute(size_t nr_shots, bool keep_states)
@@ -134,11 +186,31 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             }
             run_planned_batch(states, stream_->s, prog_, stats_.gates);
             stats_.rounds += 1;
-            std::vector<Circuit*> still;
+            // Z-basis measurements of all circuits: every marginal reduction
+            // and its copy-out enqueued, one synchronisation, then each
+            // circuit's draws (its own random stream) and collapse
+            std::vector<Circuit*> still, meas;
+            std::vector<size_t> off;
+            size_t total = 0;
+            for (size_t k = 0; k < active.size(); ++k)
+                if (more[k] && active[k]->next_is_z_measure()) {
+                    meas.push_back(active[k]);
+                    off.push_back(total);
+                    total += active[k]->measure_partials();
+                }
+            if (!meas.empty()) {
+                double* host = pinned(total);
+                for (size_t i = 0; i < meas.size(); ++i) meas[i]->measure_begin(host + off[i]);
+                if (hipStreamSynchronize(stream_->s) != hipSuccess) throw Error(ERR_HIP, "measurement reduction failed");
+                for (size_t i = 0; i < meas.size(); ++i) meas[i]->measure_end(host + off[i]);
+                stats_.sync_ops += meas.size();
+            }
             for (size_t k = 0; k < active.size(); ++k) {
                 if (more[k]) {
-                    active[k]->run_next_op();
-                    stats_.sync_ops += 1;
+                    if (std::find(meas.begin(), meas.end(), active[k]) == meas.end()) {
+                        active[k]->run_next_op();
+                        stats_.sync_ops += 1;
+                    }
                     still.push_back(active[k]);
                 } else {
                     active[k]->finish();

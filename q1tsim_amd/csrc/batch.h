@@ -32,6 +32,9 @@ public:
     // `mem_budget`: device bytes the resident states of one wave may take
     // (upper bound over their branch columns); 0 = 3/4 of free memory.
     Batch(Precision prec, int fusion_k, size_t mem_budget);
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
 
     // The circuit must stay alive until execute returns; its backend
     // precision and fusion are set to the batch's.
@@ -60,9 +63,12 @@ private:
     std::shared_ptr<SharedStream> stream_;
     DeviceBuffer prog_;
     BatchStats stats_;
+    double* host_ = nullptr;      // pinned: measurement partials of a round
+    size_t host_cap_ = 0;
 
     size_t resident_bytes(const Circuit& c, size_t nr_shots) const;
     void run(const std::vector<Circuit*>& circuits, size_t nr_shots, bool keep_states);
+    double* pinned(size_t n);
 };
 
 }  // namespace q1t

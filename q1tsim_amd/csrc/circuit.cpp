@@ -180,6 +180,42 @@ void Circuit::run_next_op()
     ++cursor_;
 }
 
+bool Circuit::next_is_z_measure() const
+{
+    if (cursor_ >= ops_.size()) return false;
+    const CircuitOp& op = ops_[cursor_];
+    return (op.type == OpType::Measure || op.type == OpType::Peek) && op.basis == 'Z';
+}
+
+void Circuit::measure_begin(double* host)
+{
+    const CircuitOp& op = ops_[cursor_];
+    state_->marginal_w0_async(op.qbit, host);
+}
+
+// The second half of measure_into (vectorstate.rs:264-275): a Binomial draw
+// per column as a count of runs with u < min(w0, 1), then the collapse.
+void Circuit::measure_end(const double* host)
+{
+    const CircuitOp& op = ops_[cursor_];
+    GpuVectorState& st = *state_;
+    if (c_state_.size() < st.nr_shots()) throw err_not_enough_space(c_state_.size(), st.nr_shots());
+    std::vector<double> w0 = st.marginal_w0_collect(op.qbit, host);
+    RngSource rng = rng_.source();
+    const std::vector<size_t>& counts = st.counts();
+    std::vector<size_t> n0s(counts.size());
+    for (size_t c = 0; c < counts.size(); ++c) {
+        const double p = w0[c] < 1.0 ? w0[c] : 1.0;
+        size_t n0 = 0;
+        for (size_t r = 0; r < counts[c]; ++r)
+            if (rng.uniform() < p) ++n0;
+        n0s[c] = n0;
+    }
+    st.measure_with_n0(op.qbit, op.cbit, n0s.data(), w0, c_state_.data(), c_state_.size(),
+                       op.type == OpType::Measure);
+    ++cursor_;
+}
+
 void Circuit::finish()
 {
     state_->flush();

@@ -62,6 +62,13 @@ public:
     void run_next_op();
     void finish();
     void release_state() { state_.reset(); }
+    // The op at the cursor is a Z-basis measure or peek: it can run in two
+    // steps, the marginal of many circuits' states enqueued before one
+    // synchronisation (batches).  measure_end completes it and advances.
+    bool next_is_z_measure() const;
+    size_t measure_partials() const { return state_->marginal_partials(); }
+    void measure_begin(double* host);
+    void measure_end(const double* host);
     size_t nr_ops() const { return ops_.size(); }
     const std::vector<CircuitOp>& ops() const { return ops_; }
     Precision precision() const { return prec_; }

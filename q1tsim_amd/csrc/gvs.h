@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 
 #include "common.h"
 #include "gates.h"
@@ -67,6 +68,31 @@ struct PendingGate {
     std::vector<int> qbits;       // index-bit positions of the listed qubits
 };
 
+// Process-wide cache of freed device blocks, reused by exact size (states of
+// a batch come and go with the same few sizes).  A released block carries an
+// event recorded on its stream: the same stream reuses it at once (stream
+// order), another stream only once the event has completed.  Cached bytes
+// are capped; on allocation failure the cache is emptied and the allocation
+// retried.
+class DeviceCache {
+public:
+    static DeviceCache& get();
+    void* alloc(size_t bytes, hipStream_t s);
+    void release(void* p, size_t bytes, hipStream_t s);
+    void trim();
+
+private:
+    struct Entry {
+        void* p;
+        size_t bytes;
+        hipStream_t s;
+        hipEvent_t ev;
+    };
+    std::vector<Entry> free_;
+    size_t cached_ = 0, cap_ = 0;
+    std::mutex mu_;
+};
+
 // A HIP stream (default priority, non-blocking) shared by the states of a
 // batch; destroyed with its last user.
 struct SharedStream {
@@ -106,6 +132,12 @@ public:
     void apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
                                 size_t nr_affected, const size_t* qbits, size_t nbits, const std::string& desc);
     std::vector<double> marginal_w0(size_t qbit);
+    // marginal_w0 in two steps (batches): enqueue the reduction and its copy
+    // into `host` (pinned, >= marginal_partials() doubles); once the stream
+    // has synchronised, collect gives exactly what marginal_w0 gives.
+    size_t marginal_partials() const;
+    void marginal_w0_async(size_t qbit, double* host);
+    std::vector<double> marginal_w0_collect(size_t qbit, const double* host) const;
     void measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
                          uint64_t* res, size_t res_len, bool collapse);
     void measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng, bool collapse);
@@ -216,6 +248,10 @@ private:
     uint64_t dim() const { return uint64_t(1) << nr_bits_; }
     void check_qbit(size_t q) const;
     void alloc_cols(size_t ncols);
+    void launch_marginal(size_t qbit, double* part);
+    std::vector<double> w0_from_partials(size_t qbit, const double* h) const;
+    void* dev_alloc(size_t bytes);
+    void dev_release(void* p, size_t bytes);
     void* small_buf(size_t bytes);
     double* partial_buf(size_t n);
     void materialize();

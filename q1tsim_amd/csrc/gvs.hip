@@ -417,6 +417,78 @@ template <typename V> static V to_v(const cdouble& c)
 
 // ------------------------------------------------------------ the state
 
+// Never destroyed: at process exit the HIP runtime may already be gone, and
+// the driver reclaims the cached blocks.
+DeviceCache& DeviceCache::get()
+{
+    static DeviceCache* c = new DeviceCache();
+    return *c;
+}
+
+void DeviceCache::trim()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    for (Entry& e : free_) {
+        HIP_OK(hipEventSynchronize(e.ev));
+        HIP_OK(hipEventDestroy(e.ev));
+        HIP_OK(hipFree(e.p));
+    }
+    free_.clear();
+    cached_ = 0;
+}
+
+void* DeviceCache::alloc(size_t bytes, hipStream_t s)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t i = 0; i < free_.size(); ++i) {
+            Entry& e = free_[i];
+            if (e.bytes != bytes) continue;
+            if (e.s != s && hipEventQuery(e.ev) != hipSuccess) continue;
+            void* p = e.p;
+            HIP_OK(hipEventDestroy(e.ev));
+            cached_ -= e.bytes;
+            free_[i] = free_.back();
+            free_.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? ERR_OUT_OF_MEMORY : ERR_HIP,
+                    "Device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void DeviceCache::release(void* p, size_t bytes, hipStream_t s)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    if (!cap_) {
+        size_t f = 0, t = 0;
+        HIP_OK(hipMemGetInfo(&f, &t));
+        cap_ = t / 8;
+    }
+    if (cached_ + bytes > cap_) {
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipFree(p));
+        return;
+    }
+    hipEvent_t ev;
+    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(ev, s));
+    free_.push_back(Entry{p, bytes, s, ev});
+    cached_ += bytes;
+}
+
 SharedStream::SharedStream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
 
 SharedStream::~SharedStream()
@@ -468,12 +540,15 @@ void GpuVectorState::set_geometry(const TileGeom& g)
 GpuVectorState::~GpuVectorState()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (amp_) (void)hipFree(amp_);
-    if (d_partial_) (void)hipFree(d_partial_);
-    if (d_tree_) (void)hipFree(d_tree_);
-    if (d_small_) (void)hipFree(d_small_);
-    for (int s = 0; s < SLOT_COUNT; ++s)
-        if (d_slot_[s]) (void)hipFree(d_slot_[s]);
+    try {
+        DeviceCache& dc = DeviceCache::get();
+        dc.release(amp_, amp_cols_ * dim() * elem_bytes(), stream_);
+        dc.release(d_partial_, partial_cap_ * sizeof(double), stream_);
+        dc.release(d_tree_, tree_cap_ * sizeof(double), stream_);
+        dc.release(d_small_, small_cap_, stream_);
+        for (int s = 0; s < SLOT_COUNT; ++s) dc.release(d_slot_[s], slot_cap_[s], stream_);
+    } catch (...) {
+    }
     for (auto& ev : ev_pool_) {
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
@@ -491,26 +566,30 @@ void* GpuVectorState::col_ptr(size_t c) const
     return static_cast<char*>(amp_) + c * dim() * elem_bytes();
 }
 
+void* GpuVectorState::dev_alloc(size_t bytes) { return DeviceCache::get().alloc(bytes, stream_); }
+
+void GpuVectorState::dev_release(void* p, size_t bytes) { DeviceCache::get().release(p, bytes, stream_); }
+
 void GpuVectorState::alloc_cols(size_t ncols)
 {
     if (ncols <= amp_cols_ && amp_) return;
-    HIP_OK(hipStreamSynchronize(stream_));
     if (amp_) {
-        HIP_OK(hipFree(amp_));
+        dev_release(amp_, amp_cols_ * dim() * elem_bytes());
         amp_ = nullptr;
         amp_cols_ = 0;
     }
-    HIP_OK(hipMalloc(&amp_, ncols * dim() * elem_bytes()));
+    amp_ = dev_alloc(ncols * dim() * elem_bytes());
     amp_cols_ = ncols;
 }
 
 void* GpuVectorState::small_buf(size_t bytes)
 {
     if (bytes > small_cap_) {
-        HIP_OK(hipStreamSynchronize(stream_));
-        if (d_small_) HIP_OK(hipFree(d_small_));
+        dev_release(d_small_, small_cap_);
+        d_small_ = nullptr;
+        small_cap_ = 0;
         size_t cap = std::max<size_t>(bytes, 1 << 20);
-        HIP_OK(hipMalloc(&d_small_, cap));
+        d_small_ = dev_alloc(cap);
         small_cap_ = cap;
     }
     return d_small_;
@@ -519,10 +598,12 @@ void* GpuVectorState::small_buf(size_t bytes)
 double* GpuVectorState::partial_buf(size_t n)
 {
     if (n > partial_cap_) {
-        HIP_OK(hipStreamSynchronize(stream_));
-        if (d_partial_) HIP_OK(hipFree(d_partial_));
-        HIP_OK(hipMalloc(&d_partial_, n * sizeof(double)));
-        partial_cap_ = n;
+        dev_release(d_partial_, partial_cap_ * sizeof(double));
+        d_partial_ = nullptr;
+        partial_cap_ = 0;
+        const size_t cap = std::max<size_t>(n, 4096);
+        d_partial_ = static_cast<double*>(dev_alloc(cap * sizeof(double)));
+        partial_cap_ = cap;
     }
     return d_partial_;
 }
@@ -809,12 +890,12 @@ static void launch_op(const GateOp& op, V* amp, uint64_t col_stride, size_t ncol
 void* GpuVectorState::scratch(int slot, size_t bytes)
 {
     if (bytes > slot_cap_[slot]) {
-        HIP_OK(hipStreamSynchronize(stream_));
-        if (d_slot_[slot]) HIP_OK(hipFree(d_slot_[slot]));
+        dev_release(d_slot_[slot], slot_cap_[slot]);
         d_slot_[slot] = nullptr;
         slot_cap_[slot] = 0;
-        const size_t cap = std::max<size_t>(bytes, 1 << 16);
-        HIP_OK(hipMalloc(&d_slot_[slot], cap));
+        size_t cap = size_t(1) << 16;
+        while (cap < bytes) cap <<= 1;
+        d_slot_[slot] = dev_alloc(cap);
         slot_cap_[slot] = cap;
     }
     return d_slot_[slot];
@@ -872,14 +953,11 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
         const size_t nr = ranges.size();
         void* old = amp_;
         const size_t old_cols = amp_cols_;
-        void* fresh = nullptr;
-        HIP_OK(hipMalloc(&fresh, nr * dim() * elem_bytes()));
+        void* fresh = dev_alloc(nr * dim() * elem_bytes());
         for (size_t r = 0; r < nr; ++r)
             HIP_OK(hipMemcpyAsync(static_cast<char*>(fresh) + r * dim() * elem_bytes(), col_ptr(ranges[r].col),
                                   dim() * elem_bytes(), hipMemcpyDeviceToDevice, stream_));
-        HIP_OK(hipStreamSynchronize(stream_));
-        HIP_OK(hipFree(old));
-        (void)old_cols;
+        dev_release(old, old_cols * dim() * elem_bytes());
         amp_ = fresh;
         amp_cols_ = nr;
     }
@@ -895,15 +973,35 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
         apply_op(op, r, e);
         r = e;
     }
-    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 // ---------------------------------------------------------- measurement
 
-std::vector<double> GpuVectorState::marginal_w0(size_t qbit)
+// Per-column partial sums of sum_{bit p = 0} |a|^2 (FP64), kNormBlocks per
+// column, into `part` (device) on the state's stream.
+static constexpr unsigned kNormBlocks = 256;
+
+void GpuVectorState::launch_marginal(size_t qbit, double* part)
 {
-    check_qbit(qbit);
-    flush();
+    const size_t nc = counts_.size();
+    const int p = int(nr_bits_ - 1 - qbit);
+    const PosList pl = make_pos({p});
+    const uint64_t nwork = dim() / 2;
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
+        if (prec_ == PREC_C128)
+            hipLaunchKernelGGL(k_norm_partial<double2>, dim3(kNormBlocks, ny), dim3(kBlock), 0, stream_,
+                               static_cast<const double2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * kNormBlocks);
+        else
+            hipLaunchKernelGGL(k_norm_partial<float2>, dim3(kNormBlocks, ny), dim3(kBlock), 0, stream_,
+                               static_cast<const float2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * kNormBlocks);
+        HIP_OK(hipGetLastError());
+    }
+}
+
+// w0 per column from the partials, summed in column-major block order.
+std::vector<double> GpuVectorState::w0_from_partials(size_t qbit, const double* h) const
+{
     const size_t nc = counts_.size();
     std::vector<double> w0(nc, 0.0);
     const int p = int(nr_bits_ - 1 - qbit);
@@ -911,26 +1009,40 @@ std::vector<double> GpuVectorState::marginal_w0(size_t qbit)
         for (size_t c = 0; c < nc; ++c) w0[c] = ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
         return w0;
     }
-    const unsigned nb = 256;
-    double* part = partial_buf(size_t(nb) * nc);
-    const PosList pl = make_pos({p});
-    const uint64_t nwork = dim() / 2;
-    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
-        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
-        if (prec_ == PREC_C128)
-            hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nb, ny), dim3(kBlock), 0, stream_,
-                               static_cast<const double2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * nb);
-        else
-            hipLaunchKernelGGL(k_norm_partial<float2>, dim3(nb, ny), dim3(kBlock), 0, stream_,
-                               static_cast<const float2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * nb);
-        HIP_OK(hipGetLastError());
-    }
-    std::vector<double> h(size_t(nb) * nc);
+    for (size_t c = 0; c < nc; ++c)
+        for (unsigned b = 0; b < kNormBlocks; ++b) w0[c] += h[c * kNormBlocks + b];
+    return w0;
+}
+
+std::vector<double> GpuVectorState::marginal_w0(size_t qbit)
+{
+    check_qbit(qbit);
+    flush();
+    if (basis_) return w0_from_partials(qbit, nullptr);
+    const size_t nc = counts_.size();
+    double* part = partial_buf(size_t(kNormBlocks) * nc);
+    launch_marginal(qbit, part);
+    std::vector<double> h(size_t(kNormBlocks) * nc);
     HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    for (size_t c = 0; c < nc; ++c)
-        for (unsigned b = 0; b < nb; ++b) w0[c] += h[c * nb + b];
-    return w0;
+    return w0_from_partials(qbit, h.data());
+}
+
+size_t GpuVectorState::marginal_partials() const { return size_t(kNormBlocks) * counts_.size(); }
+
+void GpuVectorState::marginal_w0_async(size_t qbit, double* host)
+{
+    check_qbit(qbit);
+    flush();
+    if (basis_) return;
+    double* part = partial_buf(marginal_partials());
+    launch_marginal(qbit, part);
+    HIP_OK(hipMemcpyAsync(host, part, marginal_partials() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+}
+
+std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const double* host) const
+{
+    return w0_from_partials(qbit, host);
 }
 
 void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
@@ -976,8 +1088,7 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
             ncounts.push_back(counts_[c]);
         }
     } else {
-        void* fresh = nullptr;
-        HIP_OK(hipMalloc(&fresh, new_nr * dim() * elem_bytes()));
+        void* fresh = dev_alloc(new_nr * dim() * elem_bytes());
         auto fcol = [&](size_t k) { return static_cast<char*>(fresh) + k * dim() * elem_bytes(); };
         size_t ni = 0;
         for (size_t c = 0; c < nc; ++c) {
@@ -998,13 +1109,11 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
                 ni += 2;
             }
         }
-        HIP_OK(hipStreamSynchronize(stream_));
-        HIP_OK(hipFree(amp_));
+        dev_release(amp_, amp_cols_ * dim() * elem_bytes());
         amp_ = fresh;
         amp_cols_ = new_nr;
     }
     counts_ = ncounts;
-    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng,
@@ -1039,9 +1148,10 @@ void GpuVectorState::sample_column(size_t col, const double* u01, size_t nruns, 
         td.nlev += 1;
     }
     if (total_entries > tree_cap_) {
-        HIP_OK(hipStreamSynchronize(stream_));
-        if (d_tree_) HIP_OK(hipFree(d_tree_));
-        HIP_OK(hipMalloc(&d_tree_, total_entries * sizeof(double)));
+        dev_release(d_tree_, tree_cap_ * sizeof(double));
+        d_tree_ = nullptr;
+        tree_cap_ = 0;
+        d_tree_ = static_cast<double*>(dev_alloc(total_entries * sizeof(double)));
         tree_cap_ = total_entries;
     }
     uint64_t o = 0;
