@@ -100,6 +100,9 @@ typedef struct q1t_batch_stats {
     uint64_t rounds;        /* lockstep rounds */
     uint64_t sync_ops;      /* measurements, conditional gates, resets */
     double seconds;         /* wall time of the execute call */
+    double start_s, queue_s, flush_s, measure_s, other_s;   /* its parts: state set-up, gate queueing,
+                                                               batched flushes, batched measurements,
+                                                               other ops (conditional gates, resets) */
 } q1t_batch_stats;
 /* mem_budget: device bytes one wave of states may take (0: 3/4 of free memory) */
 int q1t_batch_new(int precision, int fusion_k, size_t mem_budget, q1t_batch** out);

@@ -16,7 +16,7 @@ Batch::~Batch()
     if (host_) (void)hipHostFree(host_);
 }
 
-double* Batch::pinned(size_t n)
+char* Batch::pinned(size_t n)
 {
     if (n > host_cap_) {
         if (host_) {
@@ -25,8 +25,8 @@ double* Batch::pinned(size_t n)
             host_ = nullptr;
             host_cap_ = 0;
         }
-        const size_t cap = std::max<size_t>(n, size_t(1) << 16);
-        if (hipHostMalloc(reinterpret_cast<void**>(&host_), cap * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        const size_t cap = std::max<size_t>(n, size_t(1) << 20);
+        if (hipHostMalloc(reinterpret_cast<void**>(&host_), cap, hipHostMallocDefault) != hipSuccess)
             throw Error(ERR_OUT_OF_MEMORY, "Pinned host allocation failed");
         host_cap_ = cap;
     }
@@ -170,10 +170,17 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             ++next;
         }
         stats_.waves += 1;
+        auto tick = std::chrono::steady_clock::now();
+        auto lap = [&](double& acc) {
+            const auto now = std::chrono::steady_clock::now();
+            acc += std::chrono::duration<double>(now - tick).count();
+            tick = now;
+        };
         for (Circuit* c : wave) {
             c->start(nr_shots, stream_);
             c->state()->set_geometry(geom_);
         }
+        lap(stats_.start_s);
         std::vector<Circuit*> active = wave;
         std::vector<GpuVectorState*> states;
         std::vector<char> more;
@@ -184,27 +191,43 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
                 more[k] = active[k]->queue_gates() ? 1 : 0;
                 states.push_back(active[k]->state());
             }
+            lap(stats_.queue_s);
             run_planned_batch(states, stream_->s, prog_, stats_.gates);
             stats_.rounds += 1;
-            // Z-basis measurements of all circuits: every marginal reduction
-            // and its copy-out enqueued, one synchronisation, then each
-            // circuit's draws (its own random stream) and collapse
+            lap(stats_.flush_s);
+            // Z-basis measurements of all circuits (single-qubit marginals
+            // and measure_all sampling): everything enqueued, one
+            // synchronisation, then each circuit's draws (its own random
+            // stream), collapse and bit mapping
             std::vector<Circuit*> still, meas;
             std::vector<size_t> off;
+            std::vector<char> kind;   // 1: measure / peek, 2: measure_all / peek_all
             size_t total = 0;
-            for (size_t k = 0; k < active.size(); ++k)
-                if (more[k] && active[k]->next_is_z_measure()) {
-                    meas.push_back(active[k]);
-                    off.push_back(total);
-                    total += active[k]->measure_partials();
-                }
+            for (size_t k = 0; k < active.size(); ++k) {
+                if (!more[k]) continue;
+                Circuit* c = active[k];
+                const char kd = c->next_is_z_measure() ? 1 : c->next_is_z_measure_all() ? 2 : 0;
+                if (!kd) continue;
+                meas.push_back(c);
+                kind.push_back(kd);
+                off.push_back(total);
+                const size_t b = kd == 1 ? c->measure_partials() * sizeof(double) : c->measure_all_host_bytes();
+                total += (b + 255) / 256 * 256;
+            }
             if (!meas.empty()) {
-                double* host = pinned(total);
-                for (size_t i = 0; i < meas.size(); ++i) meas[i]->measure_begin(host + off[i]);
-                if (hipStreamSynchronize(stream_->s) != hipSuccess) throw Error(ERR_HIP, "measurement reduction failed");
-                for (size_t i = 0; i < meas.size(); ++i) meas[i]->measure_end(host + off[i]);
+                char* host = pinned(total);
+                for (size_t i = 0; i < meas.size(); ++i) {
+                    if (kind[i] == 1) meas[i]->measure_begin(reinterpret_cast<double*>(host + off[i]));
+                    else meas[i]->measure_all_begin(host + off[i]);
+                }
+                if (hipStreamSynchronize(stream_->s) != hipSuccess) throw Error(ERR_HIP, "measurement launch failed");
+                for (size_t i = 0; i < meas.size(); ++i) {
+                    if (kind[i] == 1) meas[i]->measure_end(reinterpret_cast<const double*>(host + off[i]));
+                    else meas[i]->measure_all_end(host + off[i]);
+                }
                 stats_.sync_ops += meas.size();
             }
+            lap(stats_.measure_s);
             for (size_t k = 0; k < active.size(); ++k) {
                 if (more[k]) {
                     if (std::find(meas.begin(), meas.end(), active[k]) == meas.end()) {
@@ -217,6 +240,7 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
                     if (!keep_states) active[k]->release_state();
                 }
             }
+            lap(stats_.other_s);
             active.swap(still);
         }
     }

@@ -18,6 +18,9 @@ struct BatchStats {
     uint64_t rounds = 0;       // lockstep rounds (gate flush + sync ops)
     uint64_t sync_ops = 0;     // measurements, conditional gates, resets
     double seconds = 0.0;      // wall time of execute (host clock)
+    // where that time went: state set-up, gate queueing (matrices, analysis),
+    // batched flushes (planning + kernels), batched measurements, other ops
+    double start_s = 0.0, queue_s = 0.0, flush_s = 0.0, measure_s = 0.0, other_s = 0.0;
 };
 
 // One configuration the autotuner timed.
@@ -63,12 +66,12 @@ private:
     std::shared_ptr<SharedStream> stream_;
     DeviceBuffer prog_;
     BatchStats stats_;
-    double* host_ = nullptr;      // pinned: measurement partials of a round
+    char* host_ = nullptr;        // pinned: measurement scratch of a round
     size_t host_cap_ = 0;
 
     size_t resident_bytes(const Circuit& c, size_t nr_shots) const;
     void run(const std::vector<Circuit*>& circuits, size_t nr_shots, bool keep_states);
-    double* pinned(size_t n);
+    char* pinned(size_t bytes);
 };
 
 }  // namespace q1t

@@ -385,6 +385,11 @@ extern "C" int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out)
     out->rounds = s.rounds;
     out->sync_ops = s.sync_ops;
     out->seconds = s.seconds;
+    out->start_s = s.start_s;
+    out->queue_s = s.queue_s;
+    out->flush_s = s.flush_s;
+    out->measure_s = s.measure_s;
+    out->other_s = s.other_s;
     return 0;
 }
 

@@ -216,6 +216,35 @@ void Circuit::measure_end(const double* host)
     ++cursor_;
 }
 
+bool Circuit::next_is_z_measure_all() const
+{
+    if (cursor_ >= ops_.size()) return false;
+    const CircuitOp& op = ops_[cursor_];
+    return (op.type == OpType::MeasureAll || op.type == OpType::PeekAll) && op.basis == 'Z';
+}
+
+// measure_all_into (vectorstate.rs:340-344): nr_shots uniforms from this
+// circuit's stream, then the sampling
+void Circuit::measure_all_begin(void* host)
+{
+    GpuVectorState& st = *state_;
+    const CircuitOp& op = ops_[cursor_];
+    if (c_state_.size() < st.nr_shots()) throw err_not_enough_space(c_state_.size(), st.nr_shots());
+    if (op.cbits.size() != st.nr_bits()) throw err_invalid_nr_measurement_bits(op.cbits.size(), st.nr_bits());
+    RngSource rng = rng_.source();
+    double* u = static_cast<double*>(host);
+    for (size_t r = 0; r < st.nr_shots(); ++r) u[r] = rng.uniform();
+    st.measure_all_async(host);
+}
+
+void Circuit::measure_all_end(const void* host)
+{
+    const CircuitOp& op = ops_[cursor_];
+    state_->measure_all_finish(op.cbits.data(), op.cbits.size(), host, c_state_.data(), c_state_.size(),
+                               op.type == OpType::MeasureAll);
+    ++cursor_;
+}
+
 void Circuit::finish()
 {
     state_->flush();

@@ -69,6 +69,12 @@ public:
     size_t measure_partials() const { return state_->marginal_partials(); }
     void measure_begin(double* host);
     void measure_end(const double* host);
+    // Same for a Z-basis measure_all / peek_all: begin draws this circuit's
+    // uniforms into `host` (measure_all_host_bytes) and enqueues the sampling.
+    bool next_is_z_measure_all() const;
+    size_t measure_all_host_bytes() const { return state_->measure_all_host_bytes(); }
+    void measure_all_begin(void* host);
+    void measure_all_end(const void* host);
     size_t nr_ops() const { return ops_.size(); }
     const std::vector<CircuitOp>& ops() const { return ops_; }
     Precision precision() const { return prec_; }

@@ -145,6 +145,14 @@ public:
                               size_t res_len, bool collapse);
     void measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len, RngSource& rng,
                           bool collapse);
+    // measure_all in two steps (batches): `host` (pinned, measure_all_host_bytes)
+    // starts with the nr_shots uniforms; async enqueues the sampling of every
+    // column and the copies back; once the stream has synchronised, finish
+    // gives exactly what measure_all_uniforms gives.
+    size_t measure_all_host_bytes() const;
+    void measure_all_async(void* host);
+    void measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res, size_t res_len,
+                            bool collapse);
     void reset(size_t bit, RngSource& rng);
     void reset_all();
 
@@ -256,7 +264,6 @@ private:
     double* partial_buf(size_t n);
     void materialize();
     void apply_queue_plain(size_t col_begin, size_t col_end);
-    void sample_column(size_t col, const double* u01, size_t nruns, uint64_t* out_idx);
 };
 
 }  // namespace q1t

@@ -444,7 +444,10 @@ void* DeviceCache::alloc(size_t bytes, hipStream_t s)
         for (size_t i = 0; i < free_.size(); ++i) {
             Entry& e = free_[i];
             if (e.bytes != bytes) continue;
-            if (e.s != s && hipEventQuery(e.ev) != hipSuccess) continue;
+            if (e.s != s && hipEventQuery(e.ev) != hipSuccess) {
+                (void)hipGetLastError();   // a pending event is not an error: keep it out of the next check
+                continue;
+            }
             void* p = e.p;
             HIP_OK(hipEventDestroy(e.ev));
             cached_ -= e.bytes;
@@ -1135,101 +1138,141 @@ void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_
     measure_with_n0(qbit, cbit, n0s.data(), w0, res, res_len, collapse);
 }
 
-void GpuVectorState::sample_column(size_t col, const double* u01, size_t nruns, uint64_t* out_idx)
+// Levels of the canonical tree of one column: sizes shrink by 16 until <= 16.
+static TreeDesc tree_shape(uint64_t dim, uint64_t* entries)
 {
-    // levels of the canonical tree; level sizes shrink by 16 until <= 16
     TreeDesc td{};
     td.nlev = 1;
-    td.size[0] = dim();
-    uint64_t total_entries = 0;
+    td.size[0] = dim;
+    uint64_t total = 0;
     while (td.size[td.nlev - 1] > uint64_t(kTreeFanout)) {
         td.size[td.nlev] = (td.size[td.nlev - 1] + kTreeFanout - 1) / kTreeFanout;
-        total_entries += td.size[td.nlev];
+        total += td.size[td.nlev];
         td.nlev += 1;
     }
-    if (total_entries > tree_cap_) {
+    *entries = total;
+    return td;
+}
+
+size_t GpuVectorState::measure_all_host_bytes() const
+{
+    return nr_shots_ * (sizeof(double) + sizeof(uint64_t)) + counts_.size() * 32 * sizeof(double);
+}
+
+// Host scratch of measure_all_async: [uniforms: nr_shots doubles][indices:
+// nr_shots u64][per column: 32 doubles holding the top tree level (or the
+// amplitudes of a state of at most 16 entries)].
+void GpuVectorState::measure_all_async(void* host)
+{
+    flush();
+    if (basis_) return;
+    const size_t nc = counts_.size();
+    double* hu = static_cast<double*>(host);
+    uint64_t* hidx = reinterpret_cast<uint64_t*>(hu + nr_shots_);
+    double* htop = reinterpret_cast<double*>(hidx + nr_shots_);
+    uint64_t per_col = 0;
+    TreeDesc shape = tree_shape(dim(), &per_col);
+    const size_t need = std::max<size_t>(per_col * nc, 1);
+    if (need > tree_cap_) {
         dev_release(d_tree_, tree_cap_ * sizeof(double));
         d_tree_ = nullptr;
         tree_cap_ = 0;
-        d_tree_ = static_cast<double*>(dev_alloc(total_entries * sizeof(double)));
-        tree_cap_ = total_entries;
+        d_tree_ = static_cast<double*>(dev_alloc(need * sizeof(double)));
+        tree_cap_ = need;
     }
-    uint64_t o = 0;
-    for (int l = 1; l < td.nlev; ++l) {
-        td.lev[l] = d_tree_ + o;
-        o += td.size[l];
-    }
-    const void* a = col_ptr(col);
-    for (int l = 1; l < td.nlev; ++l) {
-        const uint64_t chunks = (td.size[l - 1] + uint64_t(kBlock) * kTreeFanout - 1) / (uint64_t(kBlock) * kTreeFanout);
-        const unsigned g = unsigned(std::min<uint64_t>(chunks, 65536));
-        double* outp = const_cast<double*>(td.lev[l]);
-        if (l == 1) {
-            if (prec_ == PREC_C128)
-                hipLaunchKernelGGL(k_tree_level<double2>, dim3(g), dim3(kBlock), 0, stream_,
-                                   static_cast<const double2*>(a), td.size[0], outp);
-            else
-                hipLaunchKernelGGL(k_tree_level<float2>, dim3(g), dim3(kBlock), 0, stream_,
-                                   static_cast<const float2*>(a), td.size[0], outp);
-        } else {
-            hipLaunchKernelGGL(k_tree_level<double>, dim3(g), dim3(kBlock), 0, stream_, td.lev[l - 1], td.size[l - 1],
-                               outp);
-        }
-        HIP_OK(hipGetLastError());
-    }
-    // all-zero or NaN weights make WeightedIndex::new fail (vectorstate.rs:123-125)
-    {
-        const int top = td.nlev - 1;
-        const size_t nt = size_t(td.size[top]);
-        std::vector<double> tv(nt);
-        if (top == 0) {
-            std::vector<cdouble> av(nt);
-            if (prec_ == PREC_C128) {
-                HIP_OK(hipMemcpyAsync(av.data(), a, nt * sizeof(cdouble), hipMemcpyDeviceToHost, stream_));
-                HIP_OK(hipStreamSynchronize(stream_));
-            } else {
-                std::vector<float> f(2 * nt);
-                HIP_OK(hipMemcpyAsync(f.data(), a, f.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
-                HIP_OK(hipStreamSynchronize(stream_));
-                for (size_t k = 0; k < nt; ++k) av[k] = cdouble(f[2 * k], f[2 * k + 1]);
-            }
-            for (size_t k = 0; k < nt; ++k) tv[k] = std::norm(av[k]);
-        } else {
-            HIP_OK(hipMemcpyAsync(tv.data(), td.lev[top], nt * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));
-        }
-        double total = 0.0;
-        for (double v : tv) total += v;
-        if (!(total > 0.0) || !std::isfinite(total))
-            throw Error(ERR_INVALID_WEIGHTS, "Cannot sample from a state with zero or invalid norm");
-    }
-    // uniforms in, indices out
-    const size_t ub = nruns * sizeof(double);
+    const size_t ub = nr_shots_ * sizeof(double);
     char* buf = static_cast<char*>(small_buf(2 * ub + 64));
     double* du = reinterpret_cast<double*>(buf);
     uint64_t* dout = reinterpret_cast<uint64_t*>(buf + ub);
-    HIP_OK(hipMemcpyAsync(du, u01, ub, hipMemcpyHostToDevice, stream_));
-    const unsigned g = unsigned((nruns + kBlock - 1) / kBlock);
-    if (prec_ == PREC_C128)
-        hipLaunchKernelGGL(k_tree_sample<double2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<const double2*>(a),
-                           td, du, uint64_t(nruns), dout);
-    else
-        hipLaunchKernelGGL(k_tree_sample<float2>, dim3(g), dim3(kBlock), 0, stream_, static_cast<const float2*>(a),
-                           td, du, uint64_t(nruns), dout);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(out_idx, dout, nruns * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
-    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpyAsync(du, hu, ub, hipMemcpyHostToDevice, stream_));
+    size_t run = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        const size_t count = counts_[c];
+        if (count == 0) continue;
+        TreeDesc td = shape;
+        uint64_t o = per_col * c;
+        for (int l = 1; l < td.nlev; ++l) {
+            td.lev[l] = d_tree_ + o;
+            o += td.size[l];
+        }
+        const void* a = col_ptr(c);
+        for (int l = 1; l < td.nlev; ++l) {
+            const uint64_t chunks =
+                (td.size[l - 1] + uint64_t(kBlock) * kTreeFanout - 1) / (uint64_t(kBlock) * kTreeFanout);
+            const unsigned g = unsigned(std::min<uint64_t>(chunks, 65536));
+            double* outp = const_cast<double*>(td.lev[l]);
+            if (l == 1) {
+                if (prec_ == PREC_C128)
+                    hipLaunchKernelGGL(k_tree_level<double2>, dim3(g), dim3(kBlock), 0, stream_,
+                                       static_cast<const double2*>(a), td.size[0], outp);
+                else
+                    hipLaunchKernelGGL(k_tree_level<float2>, dim3(g), dim3(kBlock), 0, stream_,
+                                       static_cast<const float2*>(a), td.size[0], outp);
+            } else {
+                hipLaunchKernelGGL(k_tree_level<double>, dim3(g), dim3(kBlock), 0, stream_, td.lev[l - 1],
+                                   td.size[l - 1], outp);
+            }
+            HIP_OK(hipGetLastError());
+        }
+        // the top level (or the amplitudes themselves) for the norm check
+        const int top = td.nlev - 1;
+        if (top == 0)
+            HIP_OK(hipMemcpyAsync(htop + 32 * c, a, size_t(td.size[0]) * elem_bytes(), hipMemcpyDeviceToHost,
+                                  stream_));
+        else
+            HIP_OK(hipMemcpyAsync(htop + 32 * c, td.lev[top], size_t(td.size[top]) * sizeof(double),
+                                  hipMemcpyDeviceToHost, stream_));
+        // sampling is in bounds for any weights (an all-zero group walks to
+        // its first child); invalid norms are reported by measure_all_finish
+        const unsigned g = unsigned((count + kBlock - 1) / kBlock);
+        if (prec_ == PREC_C128)
+            hipLaunchKernelGGL(k_tree_sample<double2>, dim3(g), dim3(kBlock), 0, stream_,
+                               static_cast<const double2*>(a), td, du + run, uint64_t(count), dout + run);
+        else
+            hipLaunchKernelGGL(k_tree_sample<float2>, dim3(g), dim3(kBlock), 0, stream_,
+                               static_cast<const float2*>(a), td, du + run, uint64_t(count), dout + run);
+        HIP_OK(hipGetLastError());
+        run += count;
+    }
+    HIP_OK(hipMemcpyAsync(hidx, dout, nr_shots_ * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
 }
 
-void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
-                                          size_t res_len, bool collapse)
+// Grouping and bit mapping of measure_all_into_helper (vectorstate.rs:106-161)
+// on the indices measure_all_async sampled; outcomes grouped per column in
+// ascending index order.
+void GpuVectorState::measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res,
+                                        size_t res_len, bool collapse)
 {
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
-    flush();
     const size_t nc = counts_.size();
-    std::vector<uint64_t> uidx, ucnt;
-    std::vector<uint64_t> samples;
+    const uint64_t* hidx = reinterpret_cast<const uint64_t*>(static_cast<const double*>(host) + nr_shots_);
+    const double* htop = reinterpret_cast<const double*>(hidx + nr_shots_);
+    std::vector<uint64_t> uidx, ucnt, samples;
+    if (!basis_) {
+        // all-zero or NaN weights make WeightedIndex::new fail (vectorstate.rs:123-125)
+        uint64_t per_col = 0;
+        const TreeDesc td = tree_shape(dim(), &per_col);
+        const int top = td.nlev - 1;
+        for (size_t c = 0; c < nc; ++c) {
+            if (counts_[c] == 0) continue;
+            double total = 0.0;
+            const size_t nt = size_t(td.size[top]);
+            for (size_t k = 0; k < nt; ++k) {
+                if (top > 0) {
+                    total += htop[32 * c + k];
+                } else if (prec_ == PREC_C128) {
+                    const double re = htop[32 * c + 2 * k], im = htop[32 * c + 2 * k + 1];
+                    total += re * re + im * im;
+                } else {
+                    const float* f = reinterpret_cast<const float*>(htop + 32 * c);
+                    total += double(f[2 * k]) * f[2 * k] + double(f[2 * k + 1]) * f[2 * k + 1];
+                }
+            }
+            if (!(total > 0.0) || !std::isfinite(total))
+                throw Error(ERR_INVALID_WEIGHTS, "Cannot sample from a state with zero or invalid norm");
+        }
+    }
     size_t run = 0;
     for (size_t c = 0; c < nc; ++c) {
         const size_t count = counts_[c];
@@ -1240,8 +1283,7 @@ void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, co
             run += count;
             continue;
         }
-        samples.assign(count, 0);
-        sample_column(c, u01 + run, count, samples.data());
+        samples.assign(hidx + run, hidx + run + count);
         std::sort(samples.begin(), samples.end());
         for (size_t r = 0; r < count;) {
             size_t e = r;
@@ -1266,6 +1308,18 @@ void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, co
         basis_idx_ = uidx;
         counts_.assign(ucnt.begin(), ucnt.end());
     }
+}
+
+void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
+                                          size_t res_len, bool collapse)
+{
+    if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
+    if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
+    std::vector<double> host((measure_all_host_bytes() + sizeof(double) - 1) / sizeof(double));
+    std::copy(u01, u01 + nr_shots_, host.data());
+    measure_all_async(host.data());
+    HIP_OK(hipStreamSynchronize(stream_));
+    measure_all_finish(cbits, ncbits, host.data(), res, res_len, collapse);
 }
 
 void GpuVectorState::measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,

@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -821,24 +823,24 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         if (s->precision() != prec || s->geometry().M != g.M || s->geometry().R != g.R)
             throw Error(ERR_INVALID_ARGUMENT, "A batch needs one precision and tile shape");
 
-    // 1. plans and programs (host), small-state op lists
+    // 1. plans and programs (host), small-state op lists: each state on its
+    // own, in parallel over host threads; then merged into one upload
     struct Plan {
         std::vector<PlanStep> steps;
+        std::vector<std::vector<SwPhase>> ph;
+        std::vector<std::vector<SwOp>> ops;
+        std::vector<float> blocks;
+        std::vector<SmallOp> sops;
+        std::vector<double2> svals;
         std::vector<size_t> poff, ooff;
         std::vector<int> nph;
         bool small = false;
         int sop_begin = 0, sop_end = 0;
         RunStats st;
+        std::string error;
     };
     std::vector<Plan> plans(live.size());
-    std::vector<char> host;
-    std::vector<float> blocks;
-    std::vector<SmallOp> sops;
-    std::vector<double2> svals;
-    auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
-    size_t max_steps = 0;
-    int max_small_n = 0;
-    for (size_t i = 0; i < live.size(); ++i) {
+    auto plan_one = [&](size_t i) {
         GpuVectorState& st = *live[i];
         const std::vector<PendingGate>& q = st.pending();
         Plan& p = plans[i];
@@ -849,44 +851,90 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
             if (pg.op.kind == OpKind::Dense && pg.op.targets.size() > 3) lds_ok = false;
         if (lds_ok) {
             p.small = true;
-            p.sop_begin = int(sops.size());
             for (const PendingGate& pg : q)
-                if (pg.op.kind != OpKind::Identity) sops.push_back(make_small_op(pg.op, svals));
-            p.sop_end = int(sops.size());
+                if (pg.op.kind != OpKind::Identity) p.sops.push_back(make_small_op(pg.op, p.svals));
             p.st.sweeps = 1;
             p.st.launches = 1;
             p.st.gates = q.size();
             p.st.bytes = 2.0 * S;
-            max_small_n = std::max(max_small_n, n);
-            continue;
+            return;
         }
         p.steps = plan_gates(q, n, g, K);
-        p.poff.assign(p.steps.size(), 0);
-        p.ooff.assign(p.steps.size(), 0);
-        p.nph.assign(p.steps.size(), 0);
+        p.ph.resize(p.steps.size());
+        p.ops.resize(p.steps.size());
         int nblocks = 0;
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
-            std::vector<SwPhase> ph;
-            std::vector<SwOp> ops;
-            if (K) build_fused_program(q, p.steps[s], g.M, g.R, K, ph, ops, blocks, &nblocks);
-            else build_program(q, p.steps[s], g.M, g.R, ph, ops);
-            align();
-            p.poff[s] = host.size();
-            host.insert(host.end(), reinterpret_cast<const char*>(ph.data()),
-                        reinterpret_cast<const char*>(ph.data() + ph.size()));
-            align();
-            p.ooff[s] = host.size();
-            host.insert(host.end(), reinterpret_cast<const char*>(ops.data()),
-                        reinterpret_cast<const char*>(ops.data() + ops.size()));
-            p.nph[s] = int(ph.size());
+            if (K) build_fused_program(q, p.steps[s], g.M, g.R, K, p.ph[s], p.ops[s], p.blocks, &nblocks);
+            else build_program(q, p.steps[s], g.M, g.R, p.ph[s], p.ops[s]);
             p.st.sweeps += 1;
             p.st.launches += 1;
             p.st.gates += p.steps[s].gates.size();
             p.st.bytes += 2.0 * S;
-            p.st.phases += ph.size();
+            p.st.phases += p.ph[s].size();
         }
         p.st.blocks = uint64_t(nblocks);
+    };
+    {
+        const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const size_t nth = live.size() < 8 ? 1 : std::min(hw, live.size());
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i = next++; i < live.size(); i = next++) {
+                try {
+                    plan_one(i);
+                } catch (const std::exception& e) {
+                    plans[i].error = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nth; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
+        for (const Plan& p : plans)
+            if (!p.error.empty()) throw Error(ERR_INVALID_ARGUMENT, "Batch planning failed: " + p.error);
+    }
+    std::vector<char> host;
+    std::vector<float> blocks;
+    std::vector<SmallOp> sops;
+    std::vector<double2> svals;
+    auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
+    size_t max_steps = 0;
+    int max_small_n = 0;
+    for (size_t i = 0; i < live.size(); ++i) {
+        Plan& p = plans[i];
+        if (p.small) {
+            p.sop_begin = int(sops.size());
+            const uint32_t vbase = uint32_t(svals.size());
+            for (SmallOp o : p.sops) {
+                o.moff += vbase;
+                sops.push_back(o);
+            }
+            svals.insert(svals.end(), p.svals.begin(), p.svals.end());
+            p.sop_end = int(sops.size());
+            max_small_n = std::max(max_small_n, int(live[i]->nr_bits()));
+            continue;
+        }
+        const uint32_t bbase = uint32_t(blocks.size());
+        blocks.insert(blocks.end(), p.blocks.begin(), p.blocks.end());
+        p.poff.assign(p.steps.size(), 0);
+        p.ooff.assign(p.steps.size(), 0);
+        p.nph.assign(p.steps.size(), 0);
+        for (size_t s = 0; s < p.steps.size(); ++s) {
+            if (!p.steps[s].sweep) continue;
+            for (SwPhase& ph : p.ph[s])
+                if (ph.kind == 1) ph.op_begin += bbase;
+            align();
+            p.poff[s] = host.size();
+            host.insert(host.end(), reinterpret_cast<const char*>(p.ph[s].data()),
+                        reinterpret_cast<const char*>(p.ph[s].data() + p.ph[s].size()));
+            align();
+            p.ooff[s] = host.size();
+            host.insert(host.end(), reinterpret_cast<const char*>(p.ops[s].data()),
+                        reinterpret_cast<const char*>(p.ops[s].data() + p.ops[s].size()));
+            p.nph[s] = int(p.ph[s].size());
+        }
         max_steps = std::max(max_steps, p.steps.size());
     }
     align();

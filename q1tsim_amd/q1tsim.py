@@ -369,6 +369,8 @@ class Batch(object):
         _lib.check(self.__sim.q1t_batch_stats_get(self.__ptr, ctypes.byref(s)))
         g = s.gates
         return {"seconds": s.seconds, "waves": s.waves, "rounds": s.rounds, "sync_ops": s.sync_ops,
+                "start_s": s.start_s, "queue_s": s.queue_s, "flush_s": s.flush_s, "measure_s": s.measure_s,
+                "other_s": s.other_s,
                 "bytes": g.bytes, "launches": g.launches, "sweeps": g.sweeps, "gates": g.gates,
                 "blocks": g.blocks, "phases": g.phases}
 
