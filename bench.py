@@ -68,6 +68,24 @@ def roofline_of(prof):
                   "launches": launches}
 
 
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE / WRITE_SIZE, scaled by a calibration kernel), when it
+    covers this kernel and size; else None."""
+    path = os.path.join(ROOT, "profiles", "%s_layered%d_traffic.txt" % (kernel, n))
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith(" "):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("kernel") != kernel or vals.get("n_qubits") != str(n):
+        return None
+    return float(vals["traffic_bytes_per_launch"])
+
+
 def cpu_baseline(ops_full, n_full, budget_s=20.0):
     """The oracle (C restatement of VectorState, 1 core) on the first gates of
     the same circuit at n=26 (1 GiB state), extrapolated linearly in gates
@@ -226,6 +244,10 @@ def main():
     t_plain, sw_plain, bytes_plain, prof_plain = timed_steps(plain, ops, args.steps, args.warmup)
     ms_plain = 1e3 * float(np.median(t_plain))
     kname, roof = roofline_of(prof_plain)
+    roof["traffic"] = profiled_traffic(kname, n)
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = "profiles/%s_layered%d_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)" % (
+            kname, n)
 
     # fused complex64
     fused = GpuVectorState(n, 1, "c64")
