@@ -16,10 +16,12 @@ namespace q1t {
 
 TileGeom tile_geometry(Precision p)
 {
-    // 2^12-amplitude tiles, 16 amplitudes per thread, 256 threads; runs of
-    // 16 contiguous amplitudes.  Other compiled shapes: tile_shape_supported.
-    (void)p;
-    return TileGeom{12, 4, 4};
+    // 2^12-amplitude tiles in runs of 16 contiguous amplitudes.  complex128:
+    // 8 amplitudes per thread, 512 threads (117 VGPRs: 4 waves per SIMD);
+    // complex64: 16 per thread, 256 threads.  Measured on the 30-qubit
+    // layered circuit and QFT, these beat the other compiled shapes
+    // (tile_shape_supported); the batch autotuner still tries them all.
+    return p == PREC_C128 ? TileGeom{12, 3, 4} : TileGeom{12, 4, 4};
 }
 
 bool tile_shape_supported(Precision p, int M, int R)

@@ -159,3 +159,33 @@ def test_qft_sweeps():
     out = g.column(0)
     err = float(np.abs(out - np.sqrt(1 << n) * np.fft.ifft(a)).max())
     assert err < 1e-11, err
+
+
+@pytest.mark.parametrize("prec,geom", [("c128", (12, 3, 4)), ("c128", (12, 4, 4)), ("c128", (12, 4, 5)),
+                                       ("c64", (12, 4, 4)), ("c64", (13, 4, 5)), ("c64", (13, 4, 4))])
+def test_every_tile_shape(prec, geom):
+    # each compiled tile shape, with controls in and out of the tile,
+    # against the oracle (n = 16: several tiles per column)
+    n = 16
+    ops = mixed_ops(n, 120, 71)
+    amps = rand_state(n, 72)
+    g = GpuVectorState.from_amplitudes(amps, 1, prec)
+    g.set_plan(0, True)
+    g.set_geometry(*geom)
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    g.flush()
+    got = g.column(0)
+    ref = oracle(n, ops, amps)[:, 0]
+    if prec == "c128":
+        assert np.max(np.abs(got - ref)) <= 1e-12
+    else:
+        fid = abs(np.vdot(ref, got)) ** 2 / np.vdot(got, got).real
+        assert 1 - fid <= 1e-6 and np.max(np.abs(got - ref)) <= 1e-5
+    assert g.stats()["sweeps"] > 0
+
+
+def test_unsupported_tile_shape_is_rejected():
+    g = GpuVectorState(14, 1, "c128")
+    with pytest.raises(Exception):
+        g.set_geometry(13, 4, 5)
