@@ -7,12 +7,17 @@ circuit to the resident state.
 
   * plain  : complex128 state, per-gate FP64 arithmetic, gates grouped into
              LDS/register tile sweeps (the reported `value`, in HBM GB/s)
-  * fused  : complex64 state with the fused-block plan; its time, speedup
-             over plain and infidelity / max-abs error against the plain
-             complex128 state are reported alongside.
+  * fused  : complex64 state, fusion width 4: per sweep the learned cost
+             model (trained in this run on sweeps the simulator records
+             itself) picks register phases or 3-/4-qubit FP32-MFMA blocks;
+             its time, speedup over plain and infidelity / max-abs error
+             against the plain complex128 state are reported, beside greedy
+             fixed-width fusion and the per-gate complex64 path.
 
-Also reported: the 30-qubit QFT on a random normal state and FP64 sampling of
-2^16 runs (measure_all, peek semantics).  `roofline` is the dominant kernel's
+Also reported: the 30-qubit QFT on a random normal state, FP64 sampling of
+2^16 runs (measure_all, peek semantics), a batch of 4096 circuits with
+mid-circuit measurements (circuits/s), and the learned planner against greedy
+fusion on a 32-qubit random-pair circuit.  `roofline` is the dominant kernel's
 algorithmic bytes per launch over its average HIP-event duration on the
 state's stream; `cpu_baseline` times the CPU restatement of the reference
 (the oracle, one core) on a bounded sample.
@@ -155,38 +160,46 @@ def batch_extra(count, seed=2024):
     return out
 
 
-def learned_extra(n=32, depth=40, record_n=26, record_circuits=60, seed=5):
-    """Config: n-qubit depth-40 circuits with 25% CX on random pairs, complex64
-    with fused blocks.  The cost model is trained on sweeps the simulator
-    records itself (seeded exploration of the candidate programs at
-    `record_n` qubits), then the learned plan and greedy fixed-width fusion
-    run the held-out circuit; both are compared with the complex128 run."""
+def train_cost_model(record_n=26, seed=5):
+    """The learned planner's model, trained on sweeps the simulator records
+    itself: seeded exploration of the candidate programs (per-gate register
+    phases, 3- and 4-qubit blocks; both tilings) on complex64 states of
+    `record_n` qubits running random-pair (depth 40) and layered (depth 20)
+    circuits; minibatch Adam in the HIP kernels."""
     from q1tsim_amd import costmodel, workloads
     from q1tsim_amd.costmodel import CostModel
-    from q1tsim_amd.gvs import GpuVectorState, compare
     t0 = time.perf_counter()
-    circs = [workloads.random_pairs(record_n, depth, 1000 + k) for k in range(record_circuits)]
+    circs = [workloads.random_pairs(record_n, 40, 1000 + k) for k in range(40)]
+    circs += [workloads.random_layered(record_n, 20, 2000 + k) for k in range(20)]
     feat, tiles, ms = costmodel.record(record_n, circs, fusion_k=4, seed=seed)
     rec_s = time.perf_counter() - t0
     cm = CostModel(seed)
     t0 = time.perf_counter()
     mse = costmodel.fit(cm, feat, tiles, ms, epochs=40, batch=256)
-    train_s = time.perf_counter() - t0
-    blocks = int(feat[:, 1].sum() + feat[:, 2].sum())
+    info = {"record_n": record_n, "recorded_sweeps": int(len(feat)),
+            "recorded_blocks": int(feat[:, 1].sum() + feat[:, 2].sum()), "record_s": round(rec_s, 2),
+            "train_s": round(time.perf_counter() - t0, 2), "train_mse_log_time": round(mse, 4)}
+    return cm, info
+
+
+def learned_extra(cm, info, n=32, depth=40, seed=5):
+    """Config: n-qubit depth-40 circuit with 25% CX on random pairs, complex64
+    with fused blocks: the learned plan against greedy fixed-width fusion on
+    a held-out circuit, both compared with the complex128 run."""
+    from q1tsim_amd import workloads
+    from q1tsim_amd.gvs import GpuVectorState, compare
     ops = workloads.random_pairs(n, depth, seed)
     ref = GpuVectorState(n, 1, "c128")
-    ref.set_plan(0, True)
     t0 = time.perf_counter()
     apply_circuit(ref, ops)
     c128_s = time.perf_counter() - t0
-    res = {"n_qubits": n, "depth": depth, "gates": len(ops), "recorded_sweeps": int(len(feat)),
-           "recorded_blocks": blocks, "record_n": record_n, "record_s": round(rec_s, 2),
-           "train_s": round(train_s, 2), "train_mse_log_time": round(mse, 4), "c128_s": round(c128_s, 3)}
+    res = {"n_qubits": n, "depth": depth, "gates": len(ops), "c128_s": round(c128_s, 3)}
+    res.update(info)
     for mode, name in ((0, "greedy"), (1, "learned")):
         s = GpuVectorState(n, 1, "c64")
         s.set_plan(4, True)
         s.set_cost_model(cm, mode, seed)
-        apply_circuit(s, ops)   # warm-up (and planning caches)
+        apply_circuit(s, ops)   # warm-up
         s.reset_all()
         t0 = time.perf_counter()
         apply_circuit(s, ops)
@@ -207,7 +220,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--depth", type=int, default=20)
-    ap.add_argument("--fusion-k", type=int, default=5)
+    ap.add_argument("--fusion-k", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the QFT, sampling, batch and learned lines")
     ap.add_argument("--batch-circuits", type=int, default=4096)
@@ -249,13 +262,22 @@ def main():
         roof["traffic_source"] = "profiles/%s_layered%d_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)" % (
             kname, n)
 
-    # fused complex64
-    fused = GpuVectorState(n, 1, "c64")
-    fused.set_plan(args.fusion_k, True)
-    t_fused, sw_fused, bytes_fused, prof_fused = timed_steps(fused, ops, args.steps, args.warmup)
-    ms_fused = 1e3 * float(np.median(t_fused))
-    infid, maxabs = compare(plain, fused)
-    del fused
+    # complex64: the fused path as the learned planner runs it (per sweep:
+    # register phases, 3- or 4-qubit MFMA blocks), beside greedy fixed-width
+    # fusion and the per-gate c64 path
+    cm, cm_info = train_cost_model()
+    c64 = {}
+    for name, fk, mode in (("learned", args.fusion_k, 1), ("greedy_fused", args.fusion_k, 0), ("per_gate", 0, 0)):
+        st64 = GpuVectorState(n, 1, "c64")
+        st64.set_plan(fk, True)
+        st64.set_cost_model(cm, mode, 5)
+        t, sw, nb, prof = timed_steps(st64, ops, args.steps, args.warmup)
+        infid, maxabs = compare(plain, st64)
+        c64[name] = {"ms": round(1e3 * float(np.median(t)), 3), "sweeps": sw, "hbm_bytes": nb,
+                     "blocks": st64.stats()["blocks"], "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs,
+                     "kernels": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof.items()}}
+        del st64
+    ms_fused = c64["learned"]["ms"]
 
     extra = {}
     if not args.no_extra:
@@ -277,7 +299,7 @@ def main():
         except Exception as e:   # reported, not hidden: the headline line still prints
             extra["batch"] = {"error": repr(e)}
         try:
-            extra["learned_planner"] = learned_extra()
+            extra["learned_planner"] = learned_extra(cm, cm_info)
         except Exception as e:
             extra["learned_planner"] = {"error": repr(e)}
 
@@ -302,13 +324,17 @@ def main():
         "plain_c128": {"ms": round(ms_plain, 3), "sweeps": sw_plain, "hbm_bytes": bytes_plain,
                        "GBps": round(value, 1), "frac_of_peak": round(value / PEAK_GBPS, 4),
                        "unfused_equivalent_GBps": round(unf / (ms_plain * 1e-3) / 1e9, 1)},
-        "fused_c64": {"ms": round(ms_fused, 3), "sweeps": sw_fused, "hbm_bytes": bytes_fused,
-                      "GBps": round(bytes_fused / (ms_fused * 1e-3) / 1e9, 1), "fusion_k": args.fusion_k,
-                      "speedup_vs_plain": round(ms_plain / ms_fused, 3),
-                      "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs, "dtype": "f32"},
+        "fused_c64": {"ms": ms_fused, "planner": "learned (cost model trained in this run)",
+                      "sweeps": c64["learned"]["sweeps"], "blocks": c64["learned"]["blocks"],
+                      "GBps": round(c64["learned"]["hbm_bytes"] / (ms_fused * 1e-3) / 1e9, 1),
+                      "fusion_k": args.fusion_k, "speedup_vs_plain": round(ms_plain / ms_fused, 3),
+                      "infidelity_vs_c128": c64["learned"]["infidelity_vs_c128"],
+                      "max_abs_vs_c128": c64["learned"]["max_abs_vs_c128"], "dtype": "f32",
+                      "greedy_fused_ms": c64["greedy_fused"]["ms"], "per_gate_c64_ms": c64["per_gate"]["ms"],
+                      "cost_model": cm_info},
         "roofline": roof,
         "kernels_plain": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof_plain.items()},
-        "kernels_fused": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof_fused.items()},
+        "kernels_fused": c64["learned"]["kernels"],
     }
     out.update(extra)
     if not args.no_cpu:
