@@ -130,7 +130,7 @@ int q1t_gvs_flush(q1t_gvs* s);
 int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling);
 /* Build-added: tile geometry of the sweeps (M tile bits, R register bits,
  * L low bits per tile); only the compiled shapes are accepted
- * (complex128: M=12 with R=3 or 4; complex64: M=12 or 13 with R=4). */
+ * (complex128: M=12 with R=3 or 4; complex64: M=12 with R=3 or 4, M=13 with R=4). */
 int q1t_gvs_set_geometry(q1t_gvs* s, int M, int R, int L);
 int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out);
 

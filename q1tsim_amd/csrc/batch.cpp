@@ -54,6 +54,7 @@ std::vector<BatchConfig> Batch::autotune(size_t nr_shots, size_t sample)
             cand.push_back({TileGeom{12, 4, 4}, k, 0.0});
             cand.push_back({TileGeom{13, 4, 5}, k, 0.0});
         }
+        cand.push_back({TileGeom{12, 3, 4}, 0, 0.0});
     }
     sample = std::min(sample, circuits_.size());
     if (!sample) return cand;

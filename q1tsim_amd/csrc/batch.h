@@ -49,7 +49,8 @@ public:
     const BatchStats& stats() const { return stats_; }
 
     // Autotuner: execute the first `sample` circuits under every compiled
-    // tile shape of the precision and, for complex64, fusion widths {0, 4};
+    // tile shape of the precision and, for complex64, fusion widths {0, 4}
+    // (the 12/3 complex64 shape with fusion 0 only);
     // keep the fastest for execute.  Each sampled circuit's random stream is
     // restored afterwards, so execute still gives the results of
     // Circuit::execute.  Returns every configuration timed.

@@ -27,7 +27,7 @@ TileGeom tile_geometry(Precision p)
 bool tile_shape_supported(Precision p, int M, int R)
 {
     if (p == PREC_C128) return M == 12 && (R == 4 || R == 3);
-    return (M == 12 && R == 4) || (M == 13 && R == 4);
+    return (M == 12 && (R == 4 || R == 3)) || (M == 13 && R == 4);
 }
 
 static std::vector<int> mask_list(uint64_t m)
@@ -255,6 +255,7 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
         o.m[2] = op.m[1].real();
         o.m[3] = op.m[1].imag();
     }
+    for (int k = 0; k < 8; ++k) o.mf[k] = float(o.m[k]);
     return o;
 }
 

@@ -31,6 +31,7 @@ struct SwOp {
     uint32_t aux;      // SW_DENSE1 / SW_PHASE: body form; SW_BLOCK: offset of the block matrix
     uint64_t co;       // control (or phase) mask over global bits outside the tile
     double m[8];       // SW_DENSE1: m00, m01, m10, m11 (re, im); SW_PHASE: m[0..1]; SW_DIAG1: d0, d1
+    float mf[8];       // m rounded to FP32 once (the complex64 sweeps read these)
 };
 
 struct SwPhase {

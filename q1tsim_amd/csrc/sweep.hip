@@ -43,6 +43,29 @@ struct SweepArgs {
     int nphases;
 };
 
+// p[i] read through the constant address space: wave-uniform scalar loads
+// even where the kernel writes global memory (stores, DMA) before the read.
+template <typename T> __device__ __forceinline__ T load_uniform(const T* p, uint32_t i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)(p))[i];
+#else
+    return p[i];
+#endif
+}
+
+// Complex entry k of an op's values in the state's precision (complex64
+// reads the copy rounded on the host: no conversions in the op loop).
+template <typename V> __device__ __forceinline__ V opm(const SwOp& op, int k);
+template <> __device__ __forceinline__ double2 opm<double2>(const SwOp& op, int k)
+{
+    return mkc<double2>(op.m[2 * k], op.m[2 * k + 1]);
+}
+template <> __device__ __forceinline__ float2 opm<float2>(const SwOp& op, int k)
+{
+    return mkc<float2>(op.mf[2 * k], op.mf[2 * k + 1]);
+}
+
 // LDS slot of local element l: XOR-fold higher bits into the low SW bits so
 // strided phase exchanges spread over the banks (an involution).
 template <int SW, int M> __device__ __forceinline__ int swz(int l)
@@ -261,36 +284,43 @@ __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict_
 }
 
 // One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
-template <typename V, int M, int R, int SW>
-__device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& sa, uint64_t tile, V* lds)
-{
-    constexpr int NT = 1 << (M - R), NR = 1 << R;
-    using Rl = typename RealOf<V>::type;
-    const int tid = threadIdx.x;
+// Global offsets of a thread's share of a tile in natural local order:
+// element tid + k * 2^(M-R) sits at tile base | g_tid | g_k[k].
+template <int M, int R> struct TileAddr {
+    uint64_t g_tid;
+    uint64_t g_k[1 << R];
+};
 
-    // global offset of this thread's part of the natural local order
-    uint64_t g_tid = 0;
+template <int M, int R> __device__ __forceinline__ void tile_addr(const SweepArgs& sa, TileAddr<M, R>& ta)
+{
+    const int tid = threadIdx.x;
+    ta.g_tid = 0;
 #pragma unroll
     for (int b = 0; b < M - R; ++b)
-        if ((tid >> b) & 1) g_tid |= uint64_t(1) << sa.tpos[b];
-    uint64_t g_k[NR];
+        if ((tid >> b) & 1) ta.g_tid |= uint64_t(1) << sa.tpos[b];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
+    for (int k = 0; k < (1 << R); ++k) {
         uint64_t gk = 0;
 #pragma unroll
         for (int b = 0; b < R; ++b)
             if ((k >> b) & 1) gk |= uint64_t(1) << sa.tpos[M - R + b];
-        g_k[k] = gk;
+        ta.g_k[k] = gk;
     }
+}
 
+// The program's phases on a tile already in LDS, then LDS -> HBM.
+template <typename V, int M, int R, int SW>
+__device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& sa, uint64_t base, V* lds,
+                                            const TileAddr<M, R>& ta)
+{
+    constexpr int NT = 1 << (M - R), NR = 1 << R;
+    using Rl = typename RealOf<V>::type;
+    const int tid = threadIdx.x;
     {
-        const uint64_t base = deposit(tile, sa.tset);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | g_tid | g_k[k]];
-        __syncthreads();
-
         for (int p = 0; p < sa.nphases; ++p) {
-            const SwPhase& ph = sa.phases[p];
+            // the program is read through the constant address space: scalar
+            // loads, whatever else the kernel writes (including the DMA)
+            const SwPhase ph = load_uniform(sa.phases, uint32_t(p));
             if constexpr (sizeof(V) == 8) {
                 if (ph.kind == 1) {
                     // fused dense block: K = reg_mask, matrix at blocks + op_begin
@@ -320,18 +350,17 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
 #pragma unroll
             for (int k = 0; k < NR; ++k) v[k] = lds[swz<SW, M>(int(tpart | rp[k]))];
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
-                const SwOp& op = sa.ops[oi];
+                const SwOp op = load_uniform(sa.ops, oi);
                 if ((base & op.co) != op.co) continue;
                 const uint32_t cr = op.cr, ct = op.ct;
                 const bool tok = (uint32_t(tid) & ct) == ct;
                 if (op.type == SW_DENSE1) {
-                    const V m0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), m1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
-                    const V m2 = mkc<V>(Rl(op.m[4]), Rl(op.m[5])), m3 = mkc<V>(Rl(op.m[6]), Rl(op.m[7]));
+                    const V m0 = opm<V>(op, 0), m1 = opm<V>(op, 1), m2 = opm<V>(op, 2), m3 = opm<V>(op, 3);
                     dense_form_dispatch<V, NR, R>(op.aux, op.reg, v, cr, ct, tok, m0, m1, m2, m3);
                 } else if (op.type == SW_PHASE) {
-                    phase_op<V, NR>(op.aux, v, cr, ct, tok, mkc<V>(Rl(op.m[0]), Rl(op.m[1])));
+                    phase_op<V, NR>(op.aux, v, cr, ct, tok, opm<V>(op, 0));
                 } else {  // SW_DIAG1
-                    const V d0 = mkc<V>(Rl(op.m[0]), Rl(op.m[1])), d1 = mkc<V>(Rl(op.m[2]), Rl(op.m[3]));
+                    const V d0 = opm<V>(op, 0), d1 = opm<V>(op, 1);
                     diag_op<V, NR>(op, v, cr, ct, tok, base, tid, d0, d1);
                 }
             }
@@ -341,14 +370,28 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
         }
 
 #pragma unroll
-        for (int k = 0; k < NR; ++k) a[base | g_tid | g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
+        for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
         __syncthreads();
     }
 }
 
+// One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
+template <typename V, int M, int R, int SW>
+__device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& sa, uint64_t tile, V* lds)
+{
+    constexpr int NT = 1 << (M - R), NR = 1 << R;
+    TileAddr<M, R> ta;
+    tile_addr<M, R>(sa, ta);
+    const uint64_t base = deposit(tile, sa.tset);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(threadIdx.x) + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];
+    __syncthreads();
+    tile_phases<V, M, R, SW>(a, sa, base, lds, ta);
+}
+
 template <typename V, int M, int R, int SW>
 __global__ void __launch_bounds__(1 << (M - R))
-__attribute__((amdgpu_waves_per_eu(2)))
+__attribute__((amdgpu_waves_per_eu(sizeof(V) == 8 && M == 12 ? (R == 3 ? 4 : 3) : 2)))
 k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 {
     __shared__ V lds[1 << M];
@@ -601,6 +644,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
         } else {
             if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
             else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
         }
         if (st.recording() && col_end - col_begin <= 65535) {
@@ -1044,6 +1088,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
             else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
         } else {
             if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
             else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
         }
         SW_HIP_OK(hipGetLastError());

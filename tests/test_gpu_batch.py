@@ -126,7 +126,7 @@ def test_autotune_keeps_results(prec):
     for c in circs:
         batch.add(c)
     tried = batch.autotune(SHOTS, sample=8)
-    assert len(tried) == (2 if prec == "c128" else 4) and all(t[4] > 0 for t in tried)
+    assert len(tried) == (2 if prec == "c128" else 5) and all(t[4] > 0 for t in tried)
     best = min(tried, key=lambda t: t[4])
     assert batch.config() == best[:4]
     batch.execute(SHOTS)

@@ -162,7 +162,8 @@ def test_qft_sweeps():
 
 
 @pytest.mark.parametrize("prec,geom", [("c128", (12, 3, 4)), ("c128", (12, 4, 4)), ("c128", (12, 4, 5)),
-                                       ("c64", (12, 4, 4)), ("c64", (13, 4, 5)), ("c64", (13, 4, 4))])
+                                       ("c64", (12, 4, 4)), ("c64", (12, 3, 4)), ("c64", (13, 4, 5)),
+                                       ("c64", (13, 4, 4))])
 def test_every_tile_shape(prec, geom):
     # each compiled tile shape, with controls in and out of the tile,
     # against the oracle (n = 16: several tiles per column)
