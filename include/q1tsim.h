@@ -117,7 +117,7 @@ int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out);
  * configurations; *count = configurations timed. */
 typedef struct q1t_batch_config {
     int M, R, L;            /* tile bits, register bits, low bits per tile */
-    int fusion_k;           /* 0: per-gate arithmetic; 3..4: dense blocks */
+    int fusion_k;           /* 0: per-gate arithmetic; 3..6: dense blocks */
     double seconds;         /* wall time of the sample (0 for get_config) */
 } q1t_batch_config;
 int q1t_batch_autotune(q1t_batch* b, size_t nr_shots, size_t sample, q1t_batch_config* out, size_t cap,

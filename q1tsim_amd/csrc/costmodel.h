@@ -3,7 +3,7 @@
 // features of its program.  Inference and training (forward, backward,
 // gradient reduction, Adam) run in the HIP kernels of costmodel.hip.  The
 // planner uses it to choose between candidate programs of the same gates
-// (per-gate register phases, or dense blocks of 3 or 4 qubits); it never
+// (per-gate register phases, or dense blocks of 3 to 6 qubits); it never
 // changes amplitudes, only which equivalent program runs.
 #pragma once
 

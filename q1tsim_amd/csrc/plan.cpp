@@ -465,7 +465,9 @@ void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>&
         int nb;
         if (ph.kind == 1) {
             nb = int(ph.reg_mask);
-            f[nb == 3 ? 1 : 2] += 1.0f;
+            // blocks wider than 4 qubits count as 4-qubit blocks of equal work
+            if (nb == 3) f[1] += 1.0f;
+            else f[2] += float(1 << (2 * (nb - 4)));
         } else {
             nb = g.R;
             f[0] += 1.0f;

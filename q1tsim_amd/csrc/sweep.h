@@ -74,7 +74,8 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
                          int* nblocks);
 
 // Features of a sweep program for the learned cost model (16 floats):
-// register phases, blocks of 3 and of 4 qubits, one-qubit ops by body
+// register phases, blocks of 3 and of 4 qubits (a block of k > 4 qubits
+// counts as 4^(k-4) four-qubit blocks), one-qubit ops by body
 // (general, real/RX/Y, swap, phase/diagonal), ops with thread, register and
 // outer controls, precision, M, R, phases holding a local bit >= 8 in
 // registers, tile bits at positions >= 20 (long HBM strides), gates.

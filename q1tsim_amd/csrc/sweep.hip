@@ -283,6 +283,67 @@ __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict_
     }
 }
 
+// The same block product for K = 5 or 6 (2D = 64 or 128 rows): the 2D x 2D
+// real matrix (16 or 64 KiB) is too large to keep per lane, so each MFMA
+// reads its A fragment from global memory (L1/L2-resident: every tile of the
+// sweep uses the same matrix); B fragments and accumulators stay in registers.
+template <int K, int M, int SW>
+__device__ __forceinline__ void block_mfma_wide(float2* lds, const float* __restrict__ A, const SwPhase& ph, int tid)
+{
+    constexpr int D = 1 << K, D2 = 2 * D, C = 1 << (M - K);
+    constexpr int NS = D2 / 4, NRT = D2 / 16;
+    static_assert(C >= 16, "at least one 16-column chunk per tile");
+    const int lane = tid & 63, wave = tid >> 6, nwaves = int(blockDim.x) >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    uint32_t rbit[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) rbit[j] = 1u << ph.rpos[j];
+    auto rowpart = [&](int r) {
+        uint32_t p = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if ((r >> (K - 1 - j)) & 1) p |= rbit[j];
+        return p;
+    };
+    for (int chunk = wave; chunk < C / 16; chunk += nwaves) {
+        const int c = chunk * 16 + li;
+        uint32_t cp = 0;
+#pragma unroll
+        for (int b = 0; b < M - K; ++b)
+            if ((c >> b) & 1) cp |= 1u << ph.tbit[b];
+        // the K dimension in slices of 8 MFMA steps: 8 B fragments and NRT x 8
+        // A fragments in flight at a time
+        f32x4 acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int s0 = 0; s0 < NS; s0 += 8) {
+            float bv[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int rr = 4 * (s0 + s) + lg;
+                const float2 v = lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))];
+                bv[s] = rr < D ? v.x : v.y;
+            }
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) {
+                const float* arow = A + (16 * rt + li) * D2 + 4 * s0 + lg;
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * s], bv[s], acc[rt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = 16 * rt + 4 * lg + v;
+                float* dst = reinterpret_cast<float*>(&lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))]);
+                dst[rr < D ? 0 : 1] = acc[rt][v];
+            }
+    }
+}
+
 // One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
 // Global offsets of a thread's share of a tile in natural local order:
 // element tid + k * 2^(M-R) sits at tile base | g_tid | g_k[k].
@@ -308,8 +369,10 @@ template <int M, int R> __device__ __forceinline__ void tile_addr(const SweepArg
     }
 }
 
-// The program's phases on a tile already in LDS, then LDS -> HBM.
-template <typename V, int M, int R, int SW>
+// The program's phases on a tile already in LDS, then LDS -> HBM.  WIDE
+// kernels also hold the 5- and 6-qubit dense blocks; the others only 3 and 4,
+// which keeps the wide blocks' registers out of the common kernels.
+template <typename V, int M, int R, int SW, bool WIDE>
 __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& sa, uint64_t base, V* lds,
                                             const TileAddr<M, R>& ta)
 {
@@ -325,9 +388,19 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
                 if (ph.kind == 1) {
                     // fused dense block: K = reg_mask, matrix at blocks + op_begin
                     const float* A = sa.blocks + ph.op_begin;
-                    // the planner emits K = 3 or 4 only
-                    if (ph.reg_mask == 3) block_mfma<3, M, SW>(reinterpret_cast<float2*>(lds), A, ph, tid);
-                    else block_mfma<4, M, SW>(reinterpret_cast<float2*>(lds), A, ph, tid);
+                    // the planner emits K = 3 .. 6, and K > 4 only to WIDE kernels
+                    float2* l2 = reinterpret_cast<float2*>(lds);
+                    if constexpr (WIDE) {
+                        switch (ph.reg_mask) {
+                        case 3: block_mfma<3, M, SW>(l2, A, ph, tid); break;
+                        case 4: block_mfma<4, M, SW>(l2, A, ph, tid); break;
+                        case 5: block_mfma_wide<5, M, SW>(l2, A, ph, tid); break;
+                        default: block_mfma_wide<6, M, SW>(l2, A, ph, tid); break;
+                        }
+                    } else {
+                        if (ph.reg_mask == 3) block_mfma<3, M, SW>(l2, A, ph, tid);
+                        else block_mfma<4, M, SW>(l2, A, ph, tid);
+                    }
                     __syncthreads();
                     continue;
                 }
@@ -376,7 +449,7 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
 }
 
 // One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
-template <typename V, int M, int R, int SW>
+template <typename V, int M, int R, int SW, bool WIDE>
 __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& sa, uint64_t tile, V* lds)
 {
     constexpr int NT = 1 << (M - R), NR = 1 << R;
@@ -386,17 +459,17 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
 #pragma unroll
     for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(threadIdx.x) + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];
     __syncthreads();
-    tile_phases<V, M, R, SW>(a, sa, base, lds, ta);
+    tile_phases<V, M, R, SW, WIDE>(a, sa, base, lds, ta);
 }
 
-template <typename V, int M, int R, int SW>
+template <typename V, int M, int R, int SW, bool WIDE>
 __global__ void __launch_bounds__(1 << (M - R))
 __attribute__((amdgpu_waves_per_eu(sizeof(V) == 8 && M == 12 ? (R == 3 ? 4 : 3) : 2)))
 k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 {
     __shared__ V lds[1 << M];
     V* a = amp + uint64_t(blockIdx.y) * col_stride;
-    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) sweep_tile<V, M, R, SW>(a, sa, tile, lds);
+    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) sweep_tile<V, M, R, SW, WIDE>(a, sa, tile, lds);
 }
 
 // Sweeps of many states in one launch: item i covers flattened tiles
@@ -407,7 +480,7 @@ struct SweepItem {
     SweepArgs sa;
 };
 
-template <typename V, int M, int R, int SW>
+template <typename V, int M, int R, int SW, bool WIDE>
 __global__ void __launch_bounds__(1 << (M - R))
 __attribute__((amdgpu_waves_per_eu(2)))
 k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_tiles)
@@ -421,13 +494,13 @@ k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_
             else hi = mid - 1;
         }
         const SweepItem& it = items[lo];
-        sweep_tile<V, M, R, SW>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
+        sweep_tile<V, M, R, SW, WIDE>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
     }
 }
 
 template <typename V, int M, int R, int SW>
 static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
-                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce)
+                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce, bool wide)
 {
     const int n = int(st.nr_bits());
     SweepArgs sa{};
@@ -448,12 +521,30 @@ static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase*
         const unsigned ny = unsigned(std::min<size_t>(65535, ce - c0));
         const int ps = st.prof_begin();
         if (first < 0) first = ps;
-        hipLaunchKernelGGL((k_sweep<V, M, R, SW>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
-                           static_cast<V*>(st.col_ptr(c0)), dim, sa);
+        if constexpr (sizeof(V) == 8) {
+            if (wide)
+                hipLaunchKernelGGL((k_sweep<V, M, R, SW, true>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
+                                   static_cast<V*>(st.col_ptr(c0)), dim, sa);
+            else
+                hipLaunchKernelGGL((k_sweep<V, M, R, SW, false>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
+                                   static_cast<V*>(st.col_ptr(c0)), dim, sa);
+        } else {
+            hipLaunchKernelGGL((k_sweep<V, M, R, SW, false>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
+                               static_cast<V*>(st.col_ptr(c0)), dim, sa);
+        }
         SW_HIP_OK(hipGetLastError());
         st.prof_end(ps, sizeof(V) == 16 ? "k_sweep_c128" : "k_sweep_c64", 2.0 * double(dim) * sizeof(V) * ny);
     }
     return first;
+}
+
+// Whether a program holds a dense block wider than 4 qubits (launched on
+// the WIDE kernels).
+static bool has_wide_blocks(const std::vector<SwPhase>& phases)
+{
+    for (const SwPhase& ph : phases)
+        if (ph.kind == 1 && ph.reg_mask > 4) return true;
+    return false;
 }
 
 // One sweep's program: phases, ops and its own block table (block phases
@@ -465,7 +556,7 @@ struct SweepProgram {
     int nblocks = 0;
 };
 
-// kind 0: per-gate register phases; 3 or 4: dense blocks of that width
+// kind 0: per-gate register phases; 3 .. 6: dense blocks of that width
 static SweepProgram build_candidate(const std::vector<PendingGate>& q, const PlanStep& step, const TileGeom& g,
                                     int kind)
 {
@@ -484,7 +575,8 @@ static double single_gate_us(double state_bytes) { return 2.0 * state_bytes / 4.
 // Fused complex64: greedy fixed-width blocks (PLAN_GREEDY), or the cheaper of
 // two tilings -- the per-gate tiling with register programs, and the fusion
 // tiling with the cheapest of {register phases, 3-qubit blocks, 4-qubit
-// blocks} per sweep -- as ranked by the learned cost model (PLAN_LEARNED), or
+// blocks, blocks of the configured width when it is 5 or 6} per sweep -- as
+// ranked by the learned cost model (PLAN_LEARNED), or
 // a seeded random choice among the same candidates (PLAN_EXPLORE).
 static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queue, int fusion_k, double state_bytes,
                         double tiles, std::vector<PlanStep>& steps, std::vector<SweepProgram>& progs)
@@ -504,8 +596,8 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
         progs = build_all(steps, 0);
         return;
     }
-    // blocks of 3 or 4 local qubits: the MFMA stage is instantiated for those
-    const int Kg = std::min(4, std::max(3, fusion_k));
+    // blocks of 3 to 6 local qubits: block_mfma holds 3 and 4, block_mfma_wide 5 and 6
+    const int Kg = std::min(6, std::max(3, fusion_k));
     CostModel* cm = st.cost_model();
     const int mode = st.plan_mode();
     if (mode == PLAN_GREEDY || (mode == PLAN_LEARNED && (!cm || !cm->trained()))) {
@@ -514,20 +606,22 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
         return;
     }
     // candidates: tiling A (per-gate) with register programs; tiling B
-    // (fusion) with three programs per sweep
+    // (fusion) with one program per block width (0, 3, 4 and, above 4, fusion_k) per sweep
     std::vector<PlanStep> sa = plan_gates(queue, n, g, 0), sb = plan_gates(queue, n, g, Kg);
     std::vector<SweepProgram> pa = build_all(sa, 0);
-    const int kinds[3] = {0, 3, 4};
-    std::vector<std::array<SweepProgram, 3>> pb(sb.size());
+    std::vector<int> kinds = {0, 3, 4};
+    if (Kg > 4) kinds.push_back(Kg);
+    const int nk = int(kinds.size());
+    std::vector<std::vector<SweepProgram>> pb(sb.size(), std::vector<SweepProgram>(kinds.size()));
     for (size_t s = 0; s < sb.size(); ++s)
         if (sb[s].sweep)
-            for (int c = 0; c < 3; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c]);
+            for (int c = 0; c < nk; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c]);
     bool use_a = false;
     std::vector<int> pick(sb.size(), 0);
     if (mode == PLAN_EXPLORE) {
         Xoshiro256& rng = st.plan_rng();
         use_a = (rng.next() >> 63) != 0;
-        for (size_t s = 0; s < sb.size(); ++s) pick[s] = int(rng.next() % 3);
+        for (size_t s = 0; s < sb.size(); ++s) pick[s] = int(rng.next() % uint64_t(nk));
     } else {
         std::vector<float> feat;
         std::vector<std::pair<int, int>> who;   // (step of A or -1-step of B, candidate)
@@ -540,7 +634,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
             }
         for (size_t s = 0; s < sb.size(); ++s)
             if (sb[s].sweep)
-                for (int c = 0; c < 3; ++c) {
+                for (int c = 0; c < nk; ++c) {
                     sweep_features(pb[s][c].phases, pb[s][c].ops, sb[s], prec, g, f);
                     feat.insert(feat.end(), f, f + kSweepFeatures);
                     who.emplace_back(-1 - int(s), c);
@@ -637,15 +731,16 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
         const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
         const int np = int(progs[s].phases.size());
+        const bool wide = has_wide_blocks(progs[s].phases);
         int slot;
         // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
         if (st.precision() == PREC_C128) {
-            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
-            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
+            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
         } else {
-            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end);
-            else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
-            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end);
+            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
+            else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
+            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
         }
         if (st.recording() && col_end - col_begin <= 65535) {
             float f[kSweepFeatures];
@@ -862,7 +957,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
     if (live.empty()) return;
     const Precision prec = live[0]->precision();
     const TileGeom g = live[0]->geometry();
-    const int K = (prec == PREC_C64 && live[0]->fusion() > 0) ? std::min(4, std::max(3, live[0]->fusion())) : 0;
+    const int K = (prec == PREC_C64 && live[0]->fusion() > 0) ? std::min(6, std::max(3, live[0]->fusion())) : 0;
     for (GpuVectorState* s : live)
         if (s->precision() != prec || s->geometry().M != g.M || s->geometry().R != g.R)
             throw Error(ERR_INVALID_ARGUMENT, "A batch needs one precision and tile shape");
@@ -878,6 +973,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         std::vector<double2> svals;
         std::vector<size_t> poff, ooff;
         std::vector<int> nph;
+        std::vector<char> wide;   // per step: a block wider than 4 qubits
         bool small = false;
         int sop_begin = 0, sop_end = 0;
         RunStats st;
@@ -965,8 +1061,10 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         p.poff.assign(p.steps.size(), 0);
         p.ooff.assign(p.steps.size(), 0);
         p.nph.assign(p.steps.size(), 0);
+        p.wide.assign(p.steps.size(), 0);
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
+            p.wide[s] = has_wide_blocks(p.ph[s]);
             for (SwPhase& ph : p.ph[s])
                 if (ph.kind == 1) ph.op_begin += bbase;
             align();
@@ -1018,12 +1116,14 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
     std::vector<std::pair<size_t, size_t>> round_items(max_steps);   // (first item, count)
     size_t wi = 0;
     std::vector<uint64_t> round_tiles(max_steps, 0);
+    std::vector<char> round_wide(max_steps, 0);
     for (size_t r = 0; r < max_steps; ++r) {
         round_items[r].first = wi;
         uint64_t tiles = 0;
         for (size_t i = 0; i < live.size(); ++i) {
             const Plan& p = plans[i];
             if (r >= p.steps.size() || !p.steps[r].sweep) continue;
+            round_wide[r] |= p.wide[r];
             const int n = int(live[i]->nr_bits());
             for (size_t c = 0; c < live[i]->nr_cols(); ++c) {
                 SweepItem it;
@@ -1084,12 +1184,16 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         const int nit = int(round_items[r].second);
         const unsigned gx = unsigned(std::min<uint64_t>(round_tiles[r], 4096));
         if (prec == PREC_C128) {
-            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+        } else if (round_wide[r]) {
+            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, true>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
         } else {
-            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
+            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
         }
         SW_HIP_OK(hipGetLastError());
         agg.launches += 1;

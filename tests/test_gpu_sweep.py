@@ -107,7 +107,7 @@ def test_sweep_c64(n):
     assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
 
 
-@pytest.mark.parametrize("k", [3, 4, 5])
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
 def test_fused_c64_blocks(k):
     """Dense k-qubit blocks composed in FP64, applied with FP32 MFMA."""
     n = 18
