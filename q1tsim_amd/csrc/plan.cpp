@@ -86,38 +86,75 @@ static bool tile_need(const GateOp& op, int fuse_k, uint64_t* need)
     return true;
 }
 
+namespace {
+
+struct GateBits {
+    uint64_t touched;   // controls and targets
+    uint64_t need;      // bits that must be local (tile_need)
+    bool ok;            // can join a sweep
+};
+
+// Gates of `rem` (queue order) a sweep over the fixed tile `tile` takes: a
+// gate whose needed bits are local and whose bits no earlier skipped gate
+// touches; a skipped gate blocks its bits for the rest (gates on disjoint
+// bits commute).  The scan stops once every bit is blocked.
+size_t absorb_fixed(const std::vector<GateBits>& gb, const std::vector<size_t>& rem, uint64_t tile, uint64_t all,
+                    std::vector<size_t>* sel)
+{
+    uint64_t blocked = 0;
+    size_t count = 0;
+    for (size_t i : rem) {
+        const GateBits& b = gb[i];
+        if (b.ok && !(b.touched & blocked) && !(b.need & ~tile)) {
+            ++count;
+            if (sel) sel->push_back(i);
+        } else {
+            blocked |= b.touched;
+            if (blocked == all) break;
+        }
+    }
+    return count;
+}
+
+}  // namespace
+
 std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k)
 {
     std::vector<PlanStep> steps;
     std::vector<size_t> rem;
-    for (size_t i = 0; i < q.size(); ++i)
+    std::vector<GateBits> gb(q.size());
+    for (size_t i = 0; i < q.size(); ++i) {
+        gb[i].touched = touched_bits(q[i].op);
+        gb[i].ok = tile_need(q[i].op, fuse_k, &gb[i].need);
         if (q[i].op.kind != OpKind::Identity) rem.push_back(i);
+    }
     const bool tiles_ok = n >= g.M + 2;
+    const uint64_t all = n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1;
+    const uint64_t fixed = (uint64_t(1) << g.L) - 1;   // the contiguous low bits of every tile
     while (!rem.empty()) {
         if (!tiles_ok) {
             for (size_t i : rem) steps.push_back(PlanStep{false, {}, {i}});
             break;
         }
-        uint64_t tile = (uint64_t(1) << g.L) - 1;
+        // greedy start: the first gates' needed bits, in queue order
+        uint64_t tile = fixed;
         int tcount = g.L;
         uint64_t blocked = 0;
         std::vector<size_t> sel;
         for (size_t i : rem) {
-            const GateOp& op = q[i].op;
-            const uint64_t bits = touched_bits(op);
-            uint64_t need = 0;
-            if ((bits & blocked) || !tile_need(op, fuse_k, &need)) {
-                blocked |= bits;
+            const GateBits& b = gb[i];
+            if ((b.touched & blocked) || !b.ok) {
+                blocked |= b.touched;
                 continue;
             }
-            need &= ~tile;
+            const uint64_t need = b.need & ~tile;
             const int nn = __builtin_popcountll(need);
             if (tcount + nn <= g.M) {
                 tile |= need;
                 tcount += nn;
                 sel.push_back(i);
             } else {
-                blocked |= bits;
+                blocked |= b.touched;
             }
         }
         if (sel.empty()) {
@@ -133,6 +170,31 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
                     tile |= uint64_t(1) << b;
                     ++tcount;
                 }
+            // local search: exchange one free tile bit for one outside bit
+            // while that lets the sweep take more gates (first improvement;
+            // on layered brick circuits this about halves the sweeps, as the
+            // greedy start spends the tile on the first layer's targets)
+            size_t best = absorb_fixed(gb, rem, tile, all, nullptr);
+            for (int iter = 0; iter < 4 * n; ++iter) {
+                bool improved = false;
+                for (int a = g.L; a < n && !improved; ++a) {
+                    if (!((tile >> a) & 1)) continue;
+                    for (int b = 0; b < n; ++b) {
+                        if ((tile >> b) & 1) continue;
+                        const uint64_t t2 = (tile & ~(uint64_t(1) << a)) | (uint64_t(1) << b);
+                        const size_t c = absorb_fixed(gb, rem, t2, all, nullptr);
+                        if (c > best) {
+                            best = c;
+                            tile = t2;
+                            improved = true;
+                            break;
+                        }
+                    }
+                }
+                if (!improved) break;
+            }
+            sel.clear();
+            absorb_fixed(gb, rem, tile, all, &sel);
             PlanStep st;
             st.sweep = true;
             st.tile = mask_list(tile);
