@@ -405,23 +405,31 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
                     continue;
                 }
             }
-            // register bits ph.rpos[0..R-1], thread bits ph.tbit[0..M-R-1]
-            uint32_t tpart = 0;
+            // register bits ph.rpos[0..R-1] (ascending), thread bits the
+            // others: the thread's part of a local index is tid with a zero
+            // inserted at each register position, lowest first
+            uint32_t tpart = uint32_t(tid);
 #pragma unroll
-            for (int b = 0; b < M - R; ++b)
-                if ((tid >> b) & 1) tpart |= 1u << ph.tbit[b];
-            uint32_t rp[NR];   // wave-uniform local offsets of the register slots
+            for (int j = 0; j < R; ++j) {
+                const uint32_t low = (1u << ph.rpos[j]) - 1u;
+                tpart = (tpart & low) | ((tpart & ~low) << 1);
+            }
+            // the swizzle is XOR-linear and the thread and register parts
+            // have disjoint bits: slot k's LDS index = swz(tpart) ^ swz(rp[k]),
+            // the second term wave-uniform
+            const int st = swz<SW, M>(int(tpart));
+            int srp[NR];
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 uint32_t l = 0;
 #pragma unroll
                 for (int b = 0; b < R; ++b)
                     if ((k >> b) & 1) l |= 1u << ph.rpos[b];
-                rp[k] = l;
+                srp[k] = swz<SW, M>(int(l));
             }
             V v[NR];
 #pragma unroll
-            for (int k = 0; k < NR; ++k) v[k] = lds[swz<SW, M>(int(tpart | rp[k]))];
+            for (int k = 0; k < NR; ++k) v[k] = lds[st ^ srp[k]];
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
                 const SwOp op = load_uniform(sa.ops, oi);
                 if ((base & op.co) != op.co) continue;
@@ -438,7 +446,7 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
                 }
             }
 #pragma unroll
-            for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(tpart | rp[k]))] = v[k];
+            for (int k = 0; k < NR; ++k) lds[st ^ srp[k]] = v[k];
             __syncthreads();
         }
 
