@@ -324,6 +324,34 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 // One register phase by list scheduling over `pending`: a gate waits for
 // every earlier gate that shares a bit; one-qubit updates on up to R local
 // bits share the phase.  Returns the gates left for later phases.
+// Gates of `pending` a register phase over register bits `regs` takes:
+// dense gates whose target is a register bit, and diagonal gates (any bits),
+// in order; a gate not taken blocks its bits for the rest.
+size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>& pending, const LocalMap& lm,
+                    uint32_t regs, std::vector<size_t>* chosen, std::vector<size_t>* rest)
+{
+    uint64_t blocked = 0;
+    size_t count = 0;
+    for (size_t i : pending) {
+        const GateOp& op = q[i].op;
+        const uint64_t bits = touched_bits(op);
+        bool take = !(bits & blocked) && sweepable(op, nullptr);
+        if (take && op.kind == OpKind::Dense) take = (regs >> lm.local_of[op.targets[0]]) & 1;
+        if (take) {
+            ++count;
+            if (chosen) chosen->push_back(i);
+        } else {
+            blocked |= bits;
+            if (rest) rest->push_back(i);
+        }
+    }
+    return count;
+}
+
+// One register phase: R local bits held in registers.  Start from the
+// first dense targets in queue order (filled up with the lowest local bits),
+// then exchange one register bit for another local bit while the phase takes
+// more gates: every phase costs a full LDS round trip of the tile.
 std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::vector<size_t>& pending,
                                   const LocalMap& lm, int M, int R, std::vector<SwPhase>& phases,
                                   std::vector<SwOp>& ops)
@@ -331,29 +359,22 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
     uint32_t regs = 0;
     int nregs = 0;
     uint64_t blocked = 0;
-    std::vector<size_t> chosen, rest;
     for (size_t i : pending) {
         const GateOp& op = q[i].op;
         const uint64_t bits = touched_bits(op);
         if ((bits & blocked) || !sweepable(op, nullptr)) {
             blocked |= bits;
-            rest.push_back(i);
             continue;
         }
         if (op.kind == OpKind::Dense) {
             const int lb = lm.local_of[op.targets[0]];
-            if ((regs >> lb) & 1) {
-                chosen.push_back(i);
-            } else if (nregs < R) {
+            if ((regs >> lb) & 1) continue;
+            if (nregs < R) {
                 regs |= 1u << lb;
                 ++nregs;
-                chosen.push_back(i);
             } else {
                 blocked |= bits;
-                rest.push_back(i);
             }
-        } else {
-            chosen.push_back(i);
         }
     }
     for (int b = 0; b < M && nregs < R; ++b)
@@ -361,6 +382,27 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
             regs |= 1u << b;
             ++nregs;
         }
+    size_t best = phase_absorb(q, pending, lm, regs, nullptr, nullptr);
+    for (int iter = 0; iter < 4 * M; ++iter) {
+        bool improved = false;
+        for (int a = 0; a < M && !improved; ++a) {
+            if (!((regs >> a) & 1)) continue;
+            for (int b = 0; b < M; ++b) {
+                if ((regs >> b) & 1) continue;
+                const uint32_t r2 = (regs & ~(1u << a)) | (1u << b);
+                const size_t c = phase_absorb(q, pending, lm, r2, nullptr, nullptr);
+                if (c > best) {
+                    best = c;
+                    regs = r2;
+                    improved = true;
+                    break;
+                }
+            }
+        }
+        if (!improved) break;
+    }
+    std::vector<size_t> chosen, rest;
+    phase_absorb(q, pending, lm, regs, &chosen, &rest);
     SwPhase ph = make_phase(0, regs, M, uint32_t(ops.size()));
     for (size_t i : chosen) ops.push_back(make_op(q[i].op, lm, regs));
     ph.op_end = uint32_t(ops.size());

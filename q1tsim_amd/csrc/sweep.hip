@@ -88,12 +88,15 @@ template <typename V> __device__ __forceinline__ V sel(bool c, V a, V b)
 // FORM picks the body from the matrix structure, as the reference's per-gate
 // apply_mat_slice bodies do: general complex, real (H, RY), swap (X, CX),
 // [[a, ib], [ib, a]] (RX) and [[0, -i], [i, 0]] (Y).
-template <typename V, int NR, int J, int FORM, bool TC, bool RC>
+// CB >= 0: the one register control is register bit CB, known at compile
+// time (straight-line code; a run-time mask costs a branch per pair).
+template <typename V, int NR, int J, int FORM, bool TC, bool RC, int CB = -1>
 __device__ __forceinline__ void dense_pairs(V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         if ((k >> J) & 1) continue;
+        if (CB >= 0 && !((k >> CB) & 1)) continue;
         if (RC && (uint32_t(k) & cr) != cr) continue;
         const int k1 = k | (1 << J);
         const V x = v[k], y = v[k1];
@@ -125,16 +128,32 @@ __device__ __forceinline__ void dense_pairs(V (&v)[NR], uint32_t cr, bool tok, V
     }
 }
 
-template <typename V, int NR, int R, int FORM, bool TC, bool RC>
+template <typename V, int NR, int R, int FORM, bool TC, bool RC, int CB = -1>
 __device__ __forceinline__ void dense_reg(int reg, V (&v)[NR], uint32_t cr, bool tok, V m0, V m1, V m2, V m3)
 {
     switch (reg) {
-    case 0: dense_pairs<V, NR, 0, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 1: dense_pairs<V, NR, 1, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 2: dense_pairs<V, NR, 2, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
-    case 3: dense_pairs<V, NR, 3, FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3); break;
+    case 0: dense_pairs<V, NR, 0, FORM, TC, RC, CB>(v, cr, tok, m0, m1, m2, m3); break;
+    case 1: dense_pairs<V, NR, 1, FORM, TC, RC, CB>(v, cr, tok, m0, m1, m2, m3); break;
+    case 2: dense_pairs<V, NR, 2, FORM, TC, RC, CB>(v, cr, tok, m0, m1, m2, m3); break;
+    case 3: dense_pairs<V, NR, 3, FORM, TC, RC, CB>(v, cr, tok, m0, m1, m2, m3); break;
     default:
-        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM, TC, RC>(v, cr, tok, m0, m1, m2, m3);
+        if constexpr (R > 4) dense_pairs<V, NR, (R > 4 ? 4 : 0), FORM, TC, RC, CB>(v, cr, tok, m0, m1, m2, m3);
+        break;
+    }
+}
+
+// Swap-form ops (X, CX, Toffoli targets) with exactly one register control:
+// the control's register index picked at compile time.
+template <typename V, int NR, int R, bool TC>
+__device__ __forceinline__ void swap_one_rc(int cb, int reg, V (&v)[NR], bool tok)
+{
+    const V z = mkc<V>(0, 0);
+    switch (cb) {
+    case 0: dense_reg<V, NR, R, SWF_SWAP, TC, false, 0>(reg, v, 0u, tok, z, z, z, z); break;
+    case 1: dense_reg<V, NR, R, SWF_SWAP, TC, false, 1>(reg, v, 0u, tok, z, z, z, z); break;
+    case 2: dense_reg<V, NR, R, SWF_SWAP, TC, false, 2>(reg, v, 0u, tok, z, z, z, z); break;
+    default:
+        if constexpr (R > 3) dense_reg<V, NR, R, SWF_SWAP, TC, false, (R > 3 ? 3 : 0)>(reg, v, 0u, tok, z, z, z, z);
         break;
     }
 }
@@ -146,6 +165,14 @@ template <typename V, int NR, int R, int FORM>
 __device__ __forceinline__ void dense_ctl(int reg, V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, V m0, V m1, V m2,
                                           V m3)
 {
+    if constexpr (FORM == SWF_SWAP) {
+        if (cr != 0 && (cr & (cr - 1)) == 0) {
+            const int cb = __builtin_ctz(cr);
+            if (ct) swap_one_rc<V, NR, R, true>(cb, reg, v, tok);
+            else swap_one_rc<V, NR, R, false>(cb, reg, v, tok);
+            return;
+        }
+    }
     if (ct) {
         if (cr) dense_reg<V, NR, R, FORM, true, true>(reg, v, cr, tok, m0, m1, m2, m3);
         else dense_reg<V, NR, R, FORM, true, false>(reg, v, cr, tok, m0, m1, m2, m3);
@@ -170,11 +197,12 @@ __device__ __forceinline__ void dense_form_dispatch(uint32_t form, int reg, V (&
 
 // Multiply the registers whose register controls `cr` are set (and, with TC,
 // whose lane passes `tok`) by the value f(k, v[k]) gives.
-template <typename V, int NR, bool TC, bool RC, typename F>
+template <typename V, int NR, bool TC, bool RC, typename F, int CB = -1>
 __device__ __forceinline__ void scale_regs_t(V (&v)[NR], uint32_t cr, bool tok, F f)
 {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
+        if (CB >= 0 && !((k >> CB) & 1)) continue;
         if (RC && (uint32_t(k) & cr) != cr) continue;
         const V nv = f(k, v[k]);
         if constexpr (TC) v[k] = sel(tok, nv, v[k]);
@@ -182,9 +210,31 @@ __device__ __forceinline__ void scale_regs_t(V (&v)[NR], uint32_t cr, bool tok, 
     }
 }
 
+// The registers with register bit CB set (CB known at compile time).
+template <typename V, int NR, typename F>
+__device__ __forceinline__ void scale_regs_one_rc(int cb, V (&v)[NR], uint32_t ct, bool tok, F f)
+{
+#define Q1T_SCALE_CB(C)                                                     \
+    if (ct) scale_regs_t<V, NR, true, false, F, C>(v, 0u, tok, f);          \
+    else scale_regs_t<V, NR, false, false, F, C>(v, 0u, tok, f)
+    switch (cb) {
+    case 0: Q1T_SCALE_CB(0); break;
+    case 1: Q1T_SCALE_CB(1); break;
+    case 2: Q1T_SCALE_CB(2); break;
+    default:
+        if constexpr (NR > 8) Q1T_SCALE_CB((NR > 8 ? 3 : 0));
+        break;
+    }
+#undef Q1T_SCALE_CB
+}
+
 template <typename V, int NR, typename F>
 __device__ __forceinline__ void scale_regs(V (&v)[NR], uint32_t cr, uint32_t ct, bool tok, F f)
 {
+    if (cr != 0 && (cr & (cr - 1)) == 0) {
+        scale_regs_one_rc<V, NR>(__builtin_ctz(cr), v, ct, tok, f);
+        return;
+    }
     if (ct) {
         if (cr) scale_regs_t<V, NR, true, true>(v, cr, tok, f);
         else scale_regs_t<V, NR, true, false>(v, cr, tok, f);
@@ -201,7 +251,16 @@ __device__ __forceinline__ void phase_op(uint32_t form, V (&v)[NR], uint32_t cr,
     case SWP_I: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(-x.y, x.x); }); break;
     case SWP_MINUS_I: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(x.y, -x.x); }); break;
     case SWP_NEG: scale_regs<V, NR>(v, cr, ct, tok, [](int, V x) { return mkc<V>(-x.x, -x.y); }); break;
-    default: scale_regs<V, NR>(v, cr, ct, tok, [&](int, V x) { return cmul(ph1, x); }); break;
+    default:
+        if (ct) {
+            // thread controls: one per-lane factor (1 or the phase), then an
+            // unconditional product, instead of a select per register
+            const V f = sel(tok, ph1, mkc<V>(1, 0));
+            scale_regs<V, NR>(v, cr, 0u, tok, [&](int, V x) { return cmul(f, x); });
+        } else {
+            scale_regs<V, NR>(v, cr, 0u, tok, [&](int, V x) { return cmul(ph1, x); });
+        }
+        break;
     }
 }
 
