@@ -47,6 +47,34 @@ static uint64_t touched_bits(const GateOp& op)
 
 static bool is_diagonal(const GateOp& op) { return op.kind == OpKind::Diag || op.kind == OpKind::PhaseOnes; }
 
+// Bits a gate acts on non-diagonally: the targets of dense and swap gates.
+// Controls and the bits of diagonal gates act diagonally; two gates whose
+// shared bits are diagonal in both commute.
+static uint64_t nondiag_bits(const GateOp& op)
+{
+    uint64_t m = 0;
+    if (op.kind == OpKind::Dense || op.kind == OpKind::Swap2)
+        for (int t : op.targets) m |= uint64_t(1) << t;
+    return m;
+}
+
+namespace {
+
+// The gates a pass skips, as far as later gates are concerned: a later gate
+// may run before them only if it commutes with every one of them.
+struct Blocker {
+    uint64_t any = 0;   // bits the skipped gates touch
+    uint64_t nd = 0;    // bits they act on non-diagonally
+    bool blocks(uint64_t touched, uint64_t nondiag) const { return (touched & nd) || (nondiag & any); }
+    void add(uint64_t touched, uint64_t nondiag)
+    {
+        any |= touched;
+        nd |= nondiag;
+    }
+};
+
+}  // namespace
+
 bool sweepable(const GateOp& op, std::vector<int>* local_targets)
 {
     if (local_targets) local_targets->clear();
@@ -90,27 +118,27 @@ namespace {
 
 struct GateBits {
     uint64_t touched;   // controls and targets
+    uint64_t nd;        // bits acted on non-diagonally
     uint64_t need;      // bits that must be local (tile_need)
     bool ok;            // can join a sweep
 };
 
 // Gates of `rem` (queue order) a sweep over the fixed tile `tile` takes: a
-// gate whose needed bits are local and whose bits no earlier skipped gate
-// touches; a skipped gate blocks its bits for the rest (gates on disjoint
-// bits commute).  The scan stops once every bit is blocked.
+// gate whose needed bits are local and that commutes with every earlier
+// skipped gate.  The scan stops once every bit is blocked.
 size_t absorb_fixed(const std::vector<GateBits>& gb, const std::vector<size_t>& rem, uint64_t tile, uint64_t all,
                     std::vector<size_t>* sel)
 {
-    uint64_t blocked = 0;
+    Blocker blk;
     size_t count = 0;
     for (size_t i : rem) {
         const GateBits& b = gb[i];
-        if (b.ok && !(b.touched & blocked) && !(b.need & ~tile)) {
+        if (b.ok && !blk.blocks(b.touched, b.nd) && !(b.need & ~tile)) {
             ++count;
             if (sel) sel->push_back(i);
         } else {
-            blocked |= b.touched;
-            if (blocked == all) break;
+            blk.add(b.touched, b.nd);
+            if (blk.nd == all) break;
         }
     }
     return count;
@@ -118,13 +146,14 @@ size_t absorb_fixed(const std::vector<GateBits>& gb, const std::vector<size_t>& 
 
 }  // namespace
 
-std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k)
+std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k, bool search)
 {
     std::vector<PlanStep> steps;
     std::vector<size_t> rem;
     std::vector<GateBits> gb(q.size());
     for (size_t i = 0; i < q.size(); ++i) {
         gb[i].touched = touched_bits(q[i].op);
+        gb[i].nd = nondiag_bits(q[i].op);
         gb[i].ok = tile_need(q[i].op, fuse_k, &gb[i].need);
         if (q[i].op.kind != OpKind::Identity) rem.push_back(i);
     }
@@ -139,12 +168,12 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
         // greedy start: the first gates' needed bits, in queue order
         uint64_t tile = fixed;
         int tcount = g.L;
-        uint64_t blocked = 0;
+        Blocker blk;
         std::vector<size_t> sel;
         for (size_t i : rem) {
             const GateBits& b = gb[i];
-            if ((b.touched & blocked) || !b.ok) {
-                blocked |= b.touched;
+            if (blk.blocks(b.touched, b.nd) || !b.ok) {
+                blk.add(b.touched, b.nd);
                 continue;
             }
             const uint64_t need = b.need & ~tile;
@@ -154,7 +183,7 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
                 tcount += nn;
                 sel.push_back(i);
             } else {
-                blocked |= b.touched;
+                blk.add(b.touched, b.nd);
             }
         }
         if (sel.empty()) {
@@ -175,7 +204,7 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
             // on layered brick circuits this about halves the sweeps, as the
             // greedy start spends the tile on the first layer's targets)
             size_t best = absorb_fixed(gb, rem, tile, all, nullptr);
-            for (int iter = 0; iter < 4 * n; ++iter) {
+            for (int iter = 0; search && iter < 4 * n; ++iter) {
                 bool improved = false;
                 for (int a = g.L; a < n && !improved; ++a) {
                     if (!((tile >> a) & 1)) continue;
@@ -330,18 +359,18 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>& pending, const LocalMap& lm,
                     uint32_t regs, std::vector<size_t>* chosen, std::vector<size_t>* rest)
 {
-    uint64_t blocked = 0;
+    Blocker blk;
     size_t count = 0;
     for (size_t i : pending) {
         const GateOp& op = q[i].op;
-        const uint64_t bits = touched_bits(op);
-        bool take = !(bits & blocked) && sweepable(op, nullptr);
+        const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
+        bool take = !blk.blocks(bits, nd) && sweepable(op, nullptr);
         if (take && op.kind == OpKind::Dense) take = (regs >> lm.local_of[op.targets[0]]) & 1;
         if (take) {
             ++count;
             if (chosen) chosen->push_back(i);
         } else {
-            blocked |= bits;
+            blk.add(bits, nd);
             if (rest) rest->push_back(i);
         }
     }
@@ -354,16 +383,16 @@ size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>
 // more gates: every phase costs a full LDS round trip of the tile.
 std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::vector<size_t>& pending,
                                   const LocalMap& lm, int M, int R, std::vector<SwPhase>& phases,
-                                  std::vector<SwOp>& ops)
+                                  std::vector<SwOp>& ops, bool search)
 {
     uint32_t regs = 0;
     int nregs = 0;
-    uint64_t blocked = 0;
+    Blocker blk;
     for (size_t i : pending) {
         const GateOp& op = q[i].op;
-        const uint64_t bits = touched_bits(op);
-        if ((bits & blocked) || !sweepable(op, nullptr)) {
-            blocked |= bits;
+        const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
+        if (blk.blocks(bits, nd) || !sweepable(op, nullptr)) {
+            blk.add(bits, nd);
             continue;
         }
         if (op.kind == OpKind::Dense) {
@@ -373,7 +402,7 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
                 regs |= 1u << lb;
                 ++nregs;
             } else {
-                blocked |= bits;
+                blk.add(bits, nd);
             }
         }
     }
@@ -383,7 +412,7 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
             ++nregs;
         }
     size_t best = phase_absorb(q, pending, lm, regs, nullptr, nullptr);
-    for (int iter = 0; iter < 4 * M; ++iter) {
+    for (int iter = 0; search && iter < 4 * M; ++iter) {
         bool improved = false;
         for (int a = 0; a < M && !improved; ++a) {
             if (!((regs >> a) & 1)) continue;
@@ -479,16 +508,16 @@ void compose(std::vector<cdouble>& U, int nb, const GateOp& op, const int* pos)
 }  // namespace
 
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops)
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search)
 {
     const LocalMap lm(step.tile);
     std::vector<size_t> pending = step.gates;
-    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops);
+    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search);
 }
 
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
-                         int* nblocks)
+                         int* nblocks, bool search)
 {
     const LocalMap lm(step.tile);
     std::vector<size_t> pending = step.gates;
@@ -496,12 +525,12 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
         // grow one block: gates whose bits are all in the tile, while the
         // union of their local bits stays within K (order kept per bit)
         std::vector<int> B;            // local bits, B[0] = block MSB
-        uint64_t blocked = 0;
+        Blocker blk;
         std::vector<size_t> absorbed, rest;
         for (size_t i : pending) {
             const GateOp& op = q[i].op;
-            const uint64_t bits = touched_bits(op);
-            bool ok = !(bits & blocked);
+            const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
+            bool ok = !blk.blocks(bits, nd);
             std::vector<int> add;
             if (ok) {
                 for (int b = 0; b < 64 && ok; ++b)
@@ -513,7 +542,7 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
                 ok = ok && int(B.size() + add.size()) <= K;
             }
             if (!ok) {
-                blocked |= bits;
+                blk.add(bits, nd);
                 rest.push_back(i);
                 continue;
             }
@@ -521,7 +550,7 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
             absorbed.push_back(i);
         }
         if (absorbed.empty()) {
-            pending = one_reg_phase(q, pending, lm, M, R, phases, ops);
+            pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search);
             continue;
         }
         // pad to at least 3 bits (the MFMA tile needs 2D >= 16)

@@ -57,21 +57,26 @@ bool tile_shape_supported(Precision p, int M, int R);
 // Is a gate applicable inside a tile (targets that must be local, if any)?
 bool sweepable(const GateOp& op, std::vector<int>* local_targets);
 
-// Greedy plan: maximal runs of gates whose non-diagonal targets fit M tile
-// bits (the L low bits always included), respecting per-qubit order.  With
-// fuse_k > 0 the non-diagonal gates need all their bits in the tile.
-std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k);
+// Plan: runs of gates whose non-diagonal targets fit M tile bits (the L low
+// bits always included); a gate may run before earlier gates it commutes
+// with.  With fuse_k > 0 the non-diagonal gates need all their bits in the
+// tile.  `search`: refine each greedy tile (and, in the programs below, each
+// phase's register bits) by local search -- host time that pays once a
+// sweep moves hundreds of MB (kPlanSearchBytes).
+std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k,
+                                 bool search = true);
+constexpr double kPlanSearchBytes = 512.0 * 1024.0 * 1024.0;
 
 // Register phases of one sweep (per-gate arithmetic).
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops);
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search = true);
 
 // Fused program of one sweep: dense blocks of at most K local qubits
 // (composed in FP64, real-embedded, rounded to FP32 into `blocks`) and
 // register phases for the rest.
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
-                         int* nblocks);
+                         int* nblocks, bool search = true);
 
 // Features of a sweep program for the learned cost model (16 floats):
 // register phases, blocks of 3 and of 4 qubits (a block of k > 4 qubits

@@ -625,11 +625,11 @@ struct SweepProgram {
 
 // kind 0: per-gate register phases; 3 .. 6: dense blocks of that width
 static SweepProgram build_candidate(const std::vector<PendingGate>& q, const PlanStep& step, const TileGeom& g,
-                                    int kind)
+                                    int kind, bool search)
 {
     SweepProgram p;
-    if (kind == 0) build_program(q, step, g.M, g.R, p.phases, p.ops);
-    else build_fused_program(q, step, g.M, g.R, kind, p.phases, p.ops, p.blocks, &p.nblocks);
+    if (kind == 0) build_program(q, step, g.M, g.R, p.phases, p.ops, search);
+    else build_fused_program(q, step, g.M, g.R, kind, p.phases, p.ops, p.blocks, &p.nblocks, search);
     return p;
 }
 
@@ -651,15 +651,16 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     const int n = int(st.nr_bits());
     const TileGeom g = st.geometry();
     const int prec = int(st.precision());
+    const bool search = state_bytes >= kPlanSearchBytes;
     auto build_all = [&](const std::vector<PlanStep>& ss, int kind) {
         std::vector<SweepProgram> out(ss.size());
         for (size_t s = 0; s < ss.size(); ++s)
-            if (ss[s].sweep) out[s] = build_candidate(queue, ss[s], g, kind);
+            if (ss[s].sweep) out[s] = build_candidate(queue, ss[s], g, kind, search);
         return out;
     };
     const bool fused = st.precision() == PREC_C64 && fusion_k > 0;
     if (!fused) {
-        steps = plan_gates(queue, n, g, 0);
+        steps = plan_gates(queue, n, g, 0, search);
         progs = build_all(steps, 0);
         return;
     }
@@ -668,13 +669,13 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     CostModel* cm = st.cost_model();
     const int mode = st.plan_mode();
     if (mode == PLAN_GREEDY || (mode == PLAN_LEARNED && (!cm || !cm->trained()))) {
-        steps = plan_gates(queue, n, g, Kg);
+        steps = plan_gates(queue, n, g, Kg, search);
         progs = build_all(steps, Kg);
         return;
     }
     // candidates: tiling A (per-gate) with register programs; tiling B
     // (fusion) with one program per block width (0, 3, 4 and, above 4, fusion_k) per sweep
-    std::vector<PlanStep> sa = plan_gates(queue, n, g, 0), sb = plan_gates(queue, n, g, Kg);
+    std::vector<PlanStep> sa = plan_gates(queue, n, g, 0, search), sb = plan_gates(queue, n, g, Kg, search);
     std::vector<SweepProgram> pa = build_all(sa, 0);
     std::vector<int> kinds = {0, 3, 4};
     if (Kg > 4) kinds.push_back(Kg);
@@ -682,7 +683,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     std::vector<std::vector<SweepProgram>> pb(sb.size(), std::vector<SweepProgram>(kinds.size()));
     for (size_t s = 0; s < sb.size(); ++s)
         if (sb[s].sweep)
-            for (int c = 0; c < nk; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c]);
+            for (int c = 0; c < nk; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c], search);
     bool use_a = false;
     std::vector<int> pick(sb.size(), 0);
     if (mode == PLAN_EXPLORE) {
@@ -1066,14 +1067,15 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
             p.st.bytes = 2.0 * S;
             return;
         }
-        p.steps = plan_gates(q, n, g, K);
+        const bool search = S >= kPlanSearchBytes;
+        p.steps = plan_gates(q, n, g, K, search);
         p.ph.resize(p.steps.size());
         p.ops.resize(p.steps.size());
         int nblocks = 0;
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
-            if (K) build_fused_program(q, p.steps[s], g.M, g.R, K, p.ph[s], p.ops[s], p.blocks, &nblocks);
-            else build_program(q, p.steps[s], g.M, g.R, p.ph[s], p.ops[s]);
+            if (K) build_fused_program(q, p.steps[s], g.M, g.R, K, p.ph[s], p.ops[s], p.blocks, &nblocks, search);
+            else build_program(q, p.steps[s], g.M, g.R, p.ph[s], p.ops[s], search);
             p.st.sweeps += 1;
             p.st.launches += 1;
             p.st.gates += p.steps[s].gates.size();
