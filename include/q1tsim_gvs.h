@@ -128,6 +128,10 @@ int q1t_gvs_flush(q1t_gvs* s);
  * 2..6 composes gate runs into dense k-qubit blocks; tiling = 1 applies
  * runs that fit an LDS tile in one HBM sweep. */
 int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling);
+/* Build-added: local search of the sweep tiles and register phases;
+ * mode -1 (default) for states of 512 MiB and more, 0 never, 1 always.
+ * Changes only the schedule, not the gates applied. */
+int q1t_gvs_set_plan_search(q1t_gvs* s, int mode);
 /* Build-added: tile geometry of the sweeps (M tile bits, R register bits,
  * L low bits per tile); only the compiled shapes are accepted
  * (complex128: M=12 with R=3 or 4; complex64: M=12 with R=3 or 4, M=13 with R=4). */

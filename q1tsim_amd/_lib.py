@@ -135,6 +135,7 @@ SIGNATURES = {
     "q1t_gvs_read_amplitudes": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_gvs_flush": (_i32, [_vp]),
     "q1t_gvs_set_plan": (_i32, [_vp, _i32, _i32]),
+    "q1t_gvs_set_plan_search": (_i32, [_vp, _i32]),
     "q1t_gvs_set_geometry": (_i32, [_vp, _i32, _i32, _i32]),
     "q1t_gvs_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_gvs_set_profiling": (_i32, [_vp, _i32]),

@@ -643,6 +643,15 @@ extern "C" int q1t_gvs_flush(q1t_gvs* s)
     return guard([&] { s->s.flush(); });
 }
 
+extern "C" int q1t_gvs_set_plan_search(q1t_gvs* s, int mode)
+{
+    NEED(s);
+    return guard([&] {
+        if (mode < -1 || mode > 1) throw Error(ERR_INVALID_ARGUMENT, "plan search mode must be -1, 0 or 1");
+        s->s.set_plan_search(mode);
+    });
+}
+
 extern "C" int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling)
 {
     NEED(s);

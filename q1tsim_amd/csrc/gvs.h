@@ -191,6 +191,12 @@ public:
     }
     CostModel* cost_model() const { return cost_model_; }
     int plan_mode() const { return plan_mode_; }
+    // local search of tiles and register phases: -1 by state size, 0 off, 1 on
+    void set_plan_search(int mode) { plan_search_ = mode; }
+    bool plan_search(double state_bytes) const
+    {
+        return plan_search_ < 0 ? state_bytes >= 512.0 * 1024.0 * 1024.0 : plan_search_ != 0;
+    }
     Xoshiro256& plan_rng() { return plan_rng_; }
     void set_recording(bool on);
     bool recording() const { return recording_; }
@@ -235,6 +241,7 @@ private:
 
     CostModel* cost_model_ = nullptr;
     int plan_mode_ = PLAN_GREEDY;
+    int plan_search_ = -1;
     Xoshiro256 plan_rng_{1};
     bool recording_ = false;
     std::vector<SweepSample> samples_;

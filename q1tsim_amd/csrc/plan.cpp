@@ -88,6 +88,11 @@ bool sweepable(const GateOp& op, std::vector<int>* local_targets)
         if (op.targets.size() != 1) return false;
         if (local_targets) local_targets->push_back(op.targets[0]);
         return true;
+    case OpKind::Swap2:
+        // a permutation of two local bits at a phase's write-back
+        if (op.ctrl_mask) return false;
+        if (local_targets) *local_targets = op.targets;
+        return true;
     default:
         return false;
     }
@@ -189,8 +194,8 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
         if (sel.empty()) {
             // the first remaining gate cannot run inside a tile
             steps.push_back(PlanStep{false, {}, {rem.front()}});
-        } else if (sel.size() == 1 && is_diagonal(q[sel[0]].op)) {
-            // a lone diagonal gate touches only part of the state: its own kernel is cheaper
+        } else if (sel.size() == 1 && (is_diagonal(q[sel[0]].op) || q[sel[0]].op.kind == OpKind::Swap2)) {
+            // a lone diagonal or swap gate touches only part of the state: its own kernel is cheaper
             steps.push_back(PlanStep{false, {}, sel});
         } else {
             // fill the tile up to M bits with the lowest unused bits
@@ -354,18 +359,28 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 // every earlier gate that shares a bit; one-qubit updates on up to R local
 // bits share the phase.  Returns the gates left for later phases.
 // Gates of `pending` a register phase over register bits `regs` takes:
-// dense gates whose target is a register bit, and diagonal gates (any bits),
-// in order; a gate not taken blocks its bits for the rest.
+// dense gates whose target is a register bit, diagonal gates (any bits) and
+// up to kPhaseMaxSwaps swaps (applied at the write-back, so later gates on
+// their bits wait for the next phase), in order; a gate not taken blocks its
+// bits for the rest.
 size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>& pending, const LocalMap& lm,
                     uint32_t regs, std::vector<size_t>* chosen, std::vector<size_t>* rest)
 {
-    Blocker blk;
+    Blocker blk, after_swaps;
     size_t count = 0;
+    int nswap = 0;
     for (size_t i : pending) {
         const GateOp& op = q[i].op;
         const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
-        bool take = !blk.blocks(bits, nd) && sweepable(op, nullptr);
+        bool take = !blk.blocks(bits, nd) && !after_swaps.blocks(bits, nd) && sweepable(op, nullptr);
         if (take && op.kind == OpKind::Dense) take = (regs >> lm.local_of[op.targets[0]]) & 1;
+        if (take && op.kind == OpKind::Swap2) {
+            take = nswap < kPhaseMaxSwaps;
+            if (take) {
+                ++nswap;
+                after_swaps.add(bits, bits);
+            }
+        }
         if (take) {
             ++count;
             if (chosen) chosen->push_back(i);
@@ -433,7 +448,16 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
     std::vector<size_t> chosen, rest;
     phase_absorb(q, pending, lm, regs, &chosen, &rest);
     SwPhase ph = make_phase(0, regs, M, uint32_t(ops.size()));
-    for (size_t i : chosen) ops.push_back(make_op(q[i].op, lm, regs));
+    for (size_t i : chosen) {
+        const GateOp& op = q[i].op;
+        if (op.kind == OpKind::Swap2) {
+            ph.swp[2 * ph.nswap] = uint8_t(lm.local_of[op.targets[0]]);
+            ph.swp[2 * ph.nswap + 1] = uint8_t(lm.local_of[op.targets[1]]);
+            ++ph.nswap;
+        } else {
+            ops.push_back(make_op(op, lm, regs));
+        }
+    }
     ph.op_end = uint32_t(ops.size());
     phases.push_back(ph);
     return rest;

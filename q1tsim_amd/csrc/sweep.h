@@ -40,7 +40,13 @@ struct SwPhase {
     uint32_t kind;       // 0: register phase, 1: fused dense block on LDS
     uint8_t rpos[8];     // local positions of the register bits (ascending)
     uint8_t tbit[16];    // local positions of the thread bits (ascending)
+    // register phases: swaps of local bits (swp[2i], swp[2i+1]), applied in
+    // order when the phase writes the tile back to LDS (a SWAP gate costs no
+    // pass of its own: the write-back index is permuted)
+    uint8_t nswap;
+    uint8_t swp[2 * 6];
 };
+constexpr int kPhaseMaxSwaps = 6;
 
 // A planned step of the gate stream.
 struct PlanStep {
@@ -62,10 +68,9 @@ bool sweepable(const GateOp& op, std::vector<int>* local_targets);
 // with.  With fuse_k > 0 the non-diagonal gates need all their bits in the
 // tile.  `search`: refine each greedy tile (and, in the programs below, each
 // phase's register bits) by local search -- host time that pays once a
-// sweep moves hundreds of MB (kPlanSearchBytes).
+// sweep moves hundreds of MB (GpuVectorState::plan_search).
 std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k,
                                  bool search = true);
-constexpr double kPlanSearchBytes = 512.0 * 1024.0 * 1024.0;
 
 // Register phases of one sweep (per-gate arithmetic).
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,

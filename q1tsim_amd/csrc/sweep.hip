@@ -504,8 +504,36 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
                     diag_op<V, NR>(op, v, cr, ct, tok, base, tid, d0, d1);
                 }
             }
+            if (ph.nswap == 0) {
 #pragma unroll
-            for (int k = 0; k < NR; ++k) lds[st ^ srp[k]] = v[k];
+                for (int k = 0; k < NR; ++k) lds[st ^ srp[k]] = v[k];
+            } else {
+                // SWAP gates of local bits: write slot k back at its index with
+                // the listed bit pairs exchanged.  A bit permutation is linear,
+                // so it splits into a per-thread and a uniform term like the
+                // read index; other threads' slots are overwritten, so every
+                // read of the phase must be done first.
+                auto permute = [&](uint32_t l) {
+#pragma unroll
+                    for (int j = 0; j < kPhaseMaxSwaps; ++j) {
+                        if (j >= int(ph.nswap)) break;
+                        const uint32_t a = ph.swp[2 * j], b = ph.swp[2 * j + 1];
+                        const uint32_t d = ((l >> a) ^ (l >> b)) & 1u;
+                        l ^= (d << a) | (d << b);
+                    }
+                    return l;
+                };
+                const int pst = swz<SW, M>(int(permute(tpart)));
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    uint32_t l = 0;
+#pragma unroll
+                    for (int b = 0; b < R; ++b)
+                        if ((k >> b) & 1) l |= 1u << ph.rpos[b];
+                    lds[pst ^ swz<SW, M>(int(permute(l)))] = v[k];
+                }
+            }
             __syncthreads();
         }
 
@@ -651,7 +679,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     const int n = int(st.nr_bits());
     const TileGeom g = st.geometry();
     const int prec = int(st.precision());
-    const bool search = state_bytes >= kPlanSearchBytes;
+    const bool search = st.plan_search(state_bytes);
     auto build_all = [&](const std::vector<PlanStep>& ss, int kind) {
         std::vector<SweepProgram> out(ss.size());
         for (size_t s = 0; s < ss.size(); ++s)
@@ -1067,7 +1095,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
             p.st.bytes = 2.0 * S;
             return;
         }
-        const bool search = S >= kPlanSearchBytes;
+        const bool search = live[i]->plan_search(S);
         p.steps = plan_gates(q, n, g, K, search);
         p.ph.resize(p.steps.size());
         p.ops.resize(p.steps.size());

@@ -103,6 +103,11 @@ class GpuVectorState:
     def set_plan(self, fusion_k=0, tiling=True):
         check(lib().q1t_gvs_set_plan(self.h, int(fusion_k), 1 if tiling else 0))
 
+    def set_plan_search(self, mode=-1):
+        """Local search of tiles and register phases: -1 by state size
+        (512 MiB and more), 0 never, 1 always."""
+        check(lib().q1t_gvs_set_plan_search(self.h, int(mode)))
+
     def set_geometry(self, M, R, L):
         check(lib().q1t_gvs_set_geometry(self.h, int(M), int(R), int(L)))
 

@@ -190,3 +190,87 @@ def test_unsupported_tile_shape_is_rejected():
     g = GpuVectorState(14, 1, "c128")
     with pytest.raises(Exception):
         g.set_geometry(13, 4, 5)
+
+
+def swap_network(n, seed):
+    """Swaps that reverse the qubit order, then chains that share bits
+    (swap(a, b) followed by swap(b, c)), between one-qubit gates."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ops = [("swap", [j, n - 1 - j], []) for j in range(n // 2)]
+    for _ in range(40):
+        a, b, c = (int(x) for x in rng.permutation(n)[:3])
+        ops += [("swap", [a, b], []), ("h", [b], []), ("swap", [b, c], []), ("t", [a], [])]
+    return ops
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("n", [14, 17])
+def test_sweep_search_mixed(n, prec):
+    """Tiles and register phases refined by local search (on by default only
+    for states of 512 MiB and more, forced on here) and SWAP gates applied
+    as write-back permutations: amplitudes as the oracle has them."""
+    a = rand_state(n, 40 + n)
+    ops = mixed_ops(n, 250, 40 + n) + swap_network(n, n) + mixed_ops(n, 100, 41 + n)
+    g = GpuVectorState.from_amplitudes(a, 1, prec)
+    g.set_plan(0, True)
+    g.set_plan_search(1)
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    g.flush()
+    st = g.stats()
+    ref = oracle(n, ops, a)[:, 0]
+    col = g.states[:, 0]
+    err = float(np.abs(col - ref).max())
+    print("search n=%d %s: %d gates, %d sweeps, %d phases, %d launches, max-abs %.2e" %
+          (n, prec, st["gates"], st["sweeps"], st["phases"], st["launches"], err))
+    if prec == "c128":
+        assert err <= 1e-12, err
+    else:
+        fid = abs(np.vdot(ref, col)) ** 2 / float(np.vdot(col, col).real)
+        assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+
+
+def test_search_and_swaps_keep_per_gate_result():
+    """The same circuit with searched tiles, greedy tiles and per-gate
+    kernels (no tiles): complex128 results agree to rounding."""
+    n = 19
+    a = rand_state(n, 77)
+    ops = workloads.random_layered(n, 12, 77) + swap_network(n, 5) + workloads.qft(n)
+    res = {}
+    for mode, tiling in ((1, True), (0, True), (0, False)):
+        g = GpuVectorState.from_amplitudes(a, 1, "c128")
+        g.set_plan(0, tiling)
+        g.set_plan_search(mode)
+        for name, bits, params in ops:
+            g.apply_gate(orc.gate_matrix(name, params), bits)
+        g.flush()
+        res[(mode, tiling)] = (g.states[:, 0].copy(), g.stats())
+    base = res[(0, False)][0]
+    for key in ((1, True), (0, True)):
+        d = float(np.abs(res[key][0] - base).max())
+        assert d <= 1e-12, (key, d)
+    assert res[(1, True)][1]["sweeps"] <= res[(0, True)][1]["sweeps"]
+
+
+def test_fused_search_c64():
+    n = 17
+    a = rand_state(n, 9)
+    ops = workloads.random_pairs(n, 20, 9) + swap_network(n, 9)
+    g = GpuVectorState.from_amplitudes(a, 1, "c64")
+    g.set_plan(4, True)
+    g.set_plan_search(1)
+    for name, bits, params in ops:
+        g.apply_gate(orc.gate_matrix(name, params), bits)
+    g.flush()
+    ref = oracle(n, ops, a)[:, 0]
+    col = g.states[:, 0]
+    fid = abs(np.vdot(ref, col)) ** 2 / float(np.vdot(col, col).real)
+    err = float(np.abs(col - ref).max())
+    assert g.stats()["blocks"] > 0
+    assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+
+
+def test_plan_search_mode_is_checked():
+    g = GpuVectorState(12, 1, "c128")
+    with pytest.raises(Exception):
+        g.set_plan_search(2)
