@@ -358,27 +358,60 @@ SwOp make_op(const GateOp& op, const LocalMap& lm, uint32_t regs)
 // One register phase by list scheduling over `pending`: a gate waits for
 // every earlier gate that shares a bit; one-qubit updates on up to R local
 // bits share the phase.  Returns the gates left for later phases.
+// A gate the write-back map of a phase can apply: an uncontrolled SWAP, or an
+// X with at most one control, all its bits local.
+static bool writeback_gate(const GateOp& op, const LocalMap& lm)
+{
+    if (op.kind == OpKind::Swap2) {
+        if (op.ctrl_mask) return false;
+    } else if (op.kind == OpKind::Dense && op.targets.size() == 1 && op.m.size() == 4) {
+        if (dense_form(op.m.data()) != SWF_SWAP || __builtin_popcountll(op.ctrl_mask) > 1) return false;
+    } else {
+        return false;
+    }
+    for (int b = 0; b < 64; ++b)
+        if (((touched_bits(op) >> b) & 1) && lm.local_of[b] < 0) return false;
+    return true;
+}
+
+// How a register phase over `regs` runs a gate: as an op on the registers
+// (dense gates with their target in a register, diagonal gates), in the
+// write-back map (SWAP, and X / CX whose target is not a register bit), or
+// not at all.
+enum PhaseRole { ROLE_NONE, ROLE_OP, ROLE_WRITEBACK };
+static PhaseRole phase_role(const GateOp& op, const LocalMap& lm, uint32_t regs)
+{
+    if (op.kind == OpKind::Dense && op.targets.size() == 1 && lm.local_of[op.targets[0]] >= 0 &&
+        ((regs >> lm.local_of[op.targets[0]]) & 1))
+        return sweepable(op, nullptr) ? ROLE_OP : ROLE_NONE;
+    if (writeback_gate(op, lm)) return ROLE_WRITEBACK;
+    if (sweepable(op, nullptr) && op.kind != OpKind::Dense && op.kind != OpKind::Swap2) return ROLE_OP;
+    return ROLE_NONE;
+}
+
 // Gates of `pending` a register phase over register bits `regs` takes:
-// dense gates whose target is a register bit, diagonal gates (any bits) and
-// up to kPhaseMaxSwaps swaps (applied at the write-back, so later gates on
-// their bits wait for the next phase), in order; a gate not taken blocks its
-// bits for the rest.
+// write-back gates (SWAP, X, CX on local bits: applied as the phase writes
+// the tile back, so later gates on their bits wait for the next phase),
+// dense gates whose target is a register bit and diagonal gates (any bits),
+// in order; a gate not taken blocks its bits for the rest.
 size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>& pending, const LocalMap& lm,
                     uint32_t regs, std::vector<size_t>* chosen, std::vector<size_t>* rest)
 {
-    Blocker blk, after_swaps;
+    Blocker blk, after_wb;
     size_t count = 0;
-    int nswap = 0;
     for (size_t i : pending) {
         const GateOp& op = q[i].op;
         const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
-        bool take = !blk.blocks(bits, nd) && !after_swaps.blocks(bits, nd) && sweepable(op, nullptr);
-        if (take && op.kind == OpKind::Dense) take = (regs >> lm.local_of[op.targets[0]]) & 1;
-        if (take && op.kind == OpKind::Swap2) {
-            take = nswap < kPhaseMaxSwaps;
-            if (take) {
-                ++nswap;
-                after_swaps.add(bits, bits);
+        const bool free = !blk.blocks(bits, nd) && !after_wb.blocks(bits, nd);
+        bool take = false;
+        if (free) {
+            switch (phase_role(op, lm, regs)) {
+            case ROLE_OP: take = true; break;
+            case ROLE_WRITEBACK:
+                take = true;
+                after_wb.add(bits, bits);
+                break;
+            default: break;
             }
         }
         if (take) {
@@ -448,16 +481,41 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
     std::vector<size_t> chosen, rest;
     phase_absorb(q, pending, lm, regs, &chosen, &rest);
     SwPhase ph = make_phase(0, regs, M, uint32_t(ops.size()));
+    // the write-back map, composed in queue order: f(l) = XOR_j l_j col[j] ^ c
+    uint32_t col[16], cst = 0;
+    for (int j = 0; j < 16; ++j) col[j] = 1u << j;
+    auto apply = [&](auto&& g) {   // f := g o f for a map g on local indices
+        for (int j = 0; j < M; ++j) col[j] = g(col[j], false);
+        cst = g(cst, true);
+    };
     for (size_t i : chosen) {
         const GateOp& op = q[i].op;
-        if (op.kind == OpKind::Swap2) {
-            ph.swp[2 * ph.nswap] = uint8_t(lm.local_of[op.targets[0]]);
-            ph.swp[2 * ph.nswap + 1] = uint8_t(lm.local_of[op.targets[1]]);
-            ++ph.nswap;
-        } else {
+        if (phase_role(op, lm, regs) == ROLE_OP) {
             ops.push_back(make_op(op, lm, regs));
+            continue;
+        }
+        ph.wmap = 1;
+        if (op.kind == OpKind::Swap2) {
+            const int a = lm.local_of[op.targets[0]], b = lm.local_of[op.targets[1]];
+            apply([&](uint32_t x, bool) { const uint32_t d = ((x >> a) ^ (x >> b)) & 1u; return x ^ (d << a) ^ (d << b); });
+        } else {
+            const int t = lm.local_of[op.targets[0]];
+            if (op.ctrl_mask) {
+                const int c = lm.local_of[__builtin_ctzll(op.ctrl_mask)];
+                apply([&](uint32_t x, bool) { return x ^ (((x >> c) & 1u) << t); });
+            } else {
+                // X: a constant, so only the affine part changes
+                apply([&](uint32_t x, bool affine) { return affine ? x ^ (1u << t) : x; });
+            }
         }
     }
+    // columns in the kernel's order: thread bits, then register bits
+    {
+        int k = 0;
+        for (int b = 0; b < M - R; ++b) ph.wcol[k++] = uint16_t(col[ph.tbit[b]]);
+        for (int j = 0; j < R; ++j) ph.wcol[k++] = uint16_t(col[ph.rpos[j]]);
+    }
+    ph.wxor = cst;
     ph.op_end = uint32_t(ops.size());
     phases.push_back(ph);
     return rest;

@@ -40,13 +40,16 @@ struct SwPhase {
     uint32_t kind;       // 0: register phase, 1: fused dense block on LDS
     uint8_t rpos[8];     // local positions of the register bits (ascending)
     uint8_t tbit[16];    // local positions of the thread bits (ascending)
-    // register phases: swaps of local bits (swp[2i], swp[2i+1]), applied in
-    // order when the phase writes the tile back to LDS (a SWAP gate costs no
-    // pass of its own: the write-back index is permuted)
-    uint8_t nswap;
-    uint8_t swp[2 * 6];
+    // register phases: an affine map over GF(2) of the local index, applied
+    // when the phase writes the tile back to LDS -- amplitude l moves to
+    // XOR over the set bits of l of their columns, XOR wxor.  wcol holds the
+    // columns of the thread bits (tbit order), then of the register bits
+    // (rpos order).  SWAP, X and singly controlled X on local bits are such
+    // maps and cost no op of their own.
+    uint32_t wmap;       // 0: identity (plain write-back)
+    uint32_t wxor;
+    uint16_t wcol[16];
 };
-constexpr int kPhaseMaxSwaps = 6;
 
 // A planned step of the gate stream.
 struct PlanStep {

@@ -504,34 +504,29 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
                     diag_op<V, NR>(op, v, cr, ct, tok, base, tid, d0, d1);
                 }
             }
-            if (ph.nswap == 0) {
+            if (ph.wmap == 0) {
 #pragma unroll
                 for (int k = 0; k < NR; ++k) lds[st ^ srp[k]] = v[k];
             } else {
-                // SWAP gates of local bits: write slot k back at its index with
-                // the listed bit pairs exchanged.  A bit permutation is linear,
-                // so it splits into a per-thread and a uniform term like the
-                // read index; other threads' slots are overwritten, so every
-                // read of the phase must be done first.
-                auto permute = [&](uint32_t l) {
+                // SWAP / X / CX gates of local bits: slot k goes back to the
+                // affine image of its index.  The linear part splits into a
+                // per-thread term (the thread bits' columns) and a uniform
+                // term (the register bits' columns), like the read index;
+                // other threads' slots are overwritten, so every read of the
+                // phase must be done first.
+                uint32_t img = ph.wxor;
 #pragma unroll
-                    for (int j = 0; j < kPhaseMaxSwaps; ++j) {
-                        if (j >= int(ph.nswap)) break;
-                        const uint32_t a = ph.swp[2 * j], b = ph.swp[2 * j + 1];
-                        const uint32_t d = ((l >> a) ^ (l >> b)) & 1u;
-                        l ^= (d << a) | (d << b);
-                    }
-                    return l;
-                };
-                const int pst = swz<SW, M>(int(permute(tpart)));
+                for (int b = 0; b < M - R; ++b)
+                    if ((tid >> b) & 1) img ^= ph.wcol[b];
+                const int pst = swz<SW, M>(int(img));
                 __syncthreads();
 #pragma unroll
                 for (int k = 0; k < NR; ++k) {
                     uint32_t l = 0;
 #pragma unroll
                     for (int b = 0; b < R; ++b)
-                        if ((k >> b) & 1) l |= 1u << ph.rpos[b];
-                    lds[pst ^ swz<SW, M>(int(permute(l)))] = v[k];
+                        if ((k >> b) & 1) l ^= ph.wcol[M - R + b];
+                    lds[pst ^ swz<SW, M>(int(l))] = v[k];
                 }
             }
             __syncthreads();
