@@ -73,11 +73,11 @@ def roofline_of(prof):
                   "launches": launches}
 
 
-def profiled_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE / WRITE_SIZE, scaled by a calibration kernel), when it
-    covers this kernel and size; else None."""
-    path = os.path.join(ROOT, "profiles", "%s_layered%d_traffic.txt" % (kernel, n))
+def profile_values(kernel, n, what):
+    """The `key: value` lines of the committed rocprofv3 summary
+    profiles/<kernel>_layered<n>_<what>.txt when it covers this kernel and
+    size; else None."""
+    path = os.path.join(ROOT, "profiles", "%s_layered%d_%s.txt" % (kernel, n, what))
     if not os.path.exists(path):
         return None
     vals = {}
@@ -88,7 +88,14 @@ def profiled_traffic(kernel, n):
                 vals[k.strip()] = v.strip()
     if vals.get("kernel") != kernel or vals.get("n_qubits") != str(n):
         return None
-    return float(vals["traffic_bytes_per_launch"])
+    return vals
+
+
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE, scaled by a calibration kernel), or None."""
+    vals = profile_values(kernel, n, "traffic")
+    return None if vals is None else float(vals["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(ops_full, n_full, budget_s=20.0):
@@ -160,17 +167,18 @@ def batch_extra(count, seed=2024):
     return out
 
 
-def train_cost_model(record_n=26, seed=5):
+def train_cost_model(record_n=28, seed=5):
     """The learned planner's model, trained on sweeps the simulator records
     itself: seeded exploration of the candidate programs (per-gate register
     phases, 3- and 4-qubit blocks; both tilings) on complex64 states of
-    `record_n` qubits running random-pair (depth 40) and layered (depth 20)
-    circuits; minibatch Adam in the HIP kernels."""
+    `record_n` qubits (2 GiB: tiles reach the high index bits the 30- and
+    32-qubit states use) running random-pair (depth 40) and layered (depth
+    20) circuits; minibatch Adam in the HIP kernels."""
     from q1tsim_amd import costmodel, workloads
     from q1tsim_amd.costmodel import CostModel
     t0 = time.perf_counter()
-    circs = [workloads.random_pairs(record_n, 40, 1000 + k) for k in range(40)]
-    circs += [workloads.random_layered(record_n, 20, 2000 + k) for k in range(20)]
+    circs = [workloads.random_pairs(record_n, 40, 1000 + k) for k in range(20)]
+    circs += [workloads.random_layered(record_n, 20, 2000 + k) for k in range(10)]
     feat, tiles, ms = costmodel.record(record_n, circs, fusion_k=4, seed=seed)
     rec_s = time.perf_counter() - t0
     cm = CostModel(seed)
@@ -261,6 +269,14 @@ def main():
     if roof["traffic"] is not None:
         roof["traffic_source"] = "profiles/%s_layered%d_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)" % (
             kname, n)
+    # the sweep also issues FP64 VALU work for its gates; its issue time per
+    # launch (SQ_INSTS_VALU x 4 cycles / 1024 SIMDs / 2.4 GHz) next to the
+    # HBM figure says which of the two bounds the launch
+    valu = profile_values(kname, n, "valu")
+    if valu is not None:
+        issue = float(valu["valu_issue_ms_per_launch"])
+        roof["valu_issue"] = {"ms_per_launch": round(issue, 4), "frac_of_launch": round(issue / roof["avg_ms"], 4),
+                              "source": "profiles/%s_layered%d_valu.txt (rocprofv3 SQ_INSTS_VALU)" % (kname, n)}
 
     # complex64: the fused path as the learned planner runs it (per sweep:
     # register phases, 3- or 4-qubit MFMA blocks), beside greedy fixed-width

@@ -101,10 +101,14 @@ def targets(tiles, ms):
     return np.log(np.asarray(ms, dtype=np.float64) * 1e3 / (np.asarray(tiles, dtype=np.float64) / 1024.0))
 
 
-def fit(cm, feat, tiles, ms, epochs=60, batch=256, lr=2e-3, seed=1):
+def fit(cm, feat, tiles, ms, epochs=60, batch=256, lr=2e-3, seed=1, min_steps=600):
     """Standardise the features (constant ones get sigma 1), then train on
-    log(us per 1024 tiles).  Returns the final mean squared error."""
+    log(us per 1024 tiles) for `epochs` passes, raised so that there are at
+    least `min_steps` Adam steps however few the samples.  Returns the final
+    mean squared error."""
     feat = np.asarray(feat, dtype=np.float64)
+    steps_per_epoch = max(1, -(-len(feat) // batch))
+    epochs = max(epochs, -(-min_steps // steps_per_epoch))
     mu = feat.mean(0)
     sg = feat.std(0)
     sg[sg < 1e-6] = 1.0
