@@ -310,6 +310,11 @@ def main():
         plain.measure_all_into(list(range(n)), res, Xoshiro(1), collapse=False)
         extra["sample65536_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
         del plain
+        # the part's read+write streaming ceiling: a plain device copy of 8 GiB
+        from q1tsim_amd.gvs import copy_bandwidth
+        cms, cgbps = copy_bandwidth(8 << 30, 5)
+        extra["device_copy"] = {"bytes": 8 << 30, "ms": round(cms, 3), "GBps_read_plus_write": round(cgbps, 1),
+                                "frac_of_peak": round(cgbps / PEAK_GBPS, 4)}
         try:
             extra["batch"] = batch_extra(args.batch_circuits)
         except Exception as e:   # reported, not hidden: the headline line still prints

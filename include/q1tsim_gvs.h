@@ -148,6 +148,9 @@ typedef struct {
 } q1t_kernel_profile;
 int q1t_gvs_set_profiling(q1t_gvs* s, int on);
 int q1t_gvs_kernel_profile(const q1t_gvs* s, q1t_kernel_profile* out, size_t cap, size_t* count);
+/* Build-added measurement: best-of-`reps` time of a device-to-device copy of
+ * `bytes` (HIP events), i.e. the read+write streaming rate of the part. */
+int q1t_copy_bandwidth(size_t bytes, int reps, double* best_ms);
 
 /* Build-added accuracy report: <a|b>, max_i |a_i - b_i| and both squared
  * norms over one column of each state, FP64 on the device (either state may

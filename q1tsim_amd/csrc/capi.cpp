@@ -694,6 +694,15 @@ extern "C" int q1t_gvs_set_profiling(q1t_gvs* s, int on)
     return guard([&] { s->s.set_profiling(on != 0); });
 }
 
+extern "C" int q1t_copy_bandwidth(size_t bytes, int reps, double* best_ms)
+{
+    NEED(best_ms);
+    return guard([&] {
+        if (bytes == 0 || reps < 1) throw Error(ERR_INVALID_ARGUMENT, "bytes and reps must be positive");
+        *best_ms = measure_copy_ms(bytes, reps);
+    });
+}
+
 extern "C" int q1t_gvs_kernel_profile(const q1t_gvs* s, q1t_kernel_profile* out, size_t cap, size_t* count)
 {
     NEED(s);

@@ -103,6 +103,9 @@ struct SharedStream {
     SharedStream& operator=(const SharedStream&) = delete;
 };
 
+// Best-of-`reps` time (ms) of a device-to-device copy of `bytes`.
+double measure_copy_ms(size_t bytes, int reps);
+
 class GpuVectorState {
 public:
     // `shared`: run on a stream shared with other states (batches) instead

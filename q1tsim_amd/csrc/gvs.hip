@@ -1536,4 +1536,45 @@ StateComparison GpuVectorState::compare(GpuVectorState& a, size_t col_a, GpuVect
     return r;
 }
 
+// Device-to-device copy of `bytes` (hipMemcpyAsync on a fresh stream),
+// best of `reps` timed with HIP events: the read+write streaming rate a
+// full pass over HBM can reach on this part.
+double measure_copy_ms(size_t bytes, int reps)
+{
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best = 1e300;
+    auto release = [&] {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    };
+    try {
+        HIP_OK(hipMalloc(&a, bytes));
+        HIP_OK(hipMalloc(&b, bytes));
+        HIP_OK(hipStreamCreate(&st));
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        HIP_OK(hipMemsetAsync(a, 0, bytes, st));
+        HIP_OK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st));   // first touch
+        for (int r = 0; r < reps; ++r) {
+            HIP_OK(hipEventRecord(e0, st));
+            HIP_OK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipEventRecord(e1, st));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, double(ms));
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return best;
+}
+
 }  // namespace q1t

@@ -42,6 +42,14 @@ def gate_matrix(name, params=()):
     return m
 
 
+def copy_bandwidth(nbytes=8 << 30, reps=5):
+    """Device-to-device copy of `nbytes`, best of `reps` (HIP events):
+    (ms, GB/s counting the bytes read and written)."""
+    ms = ctypes.c_double(0.0)
+    check(lib().q1t_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
 def compare(a, b, col_a=0, col_b=0):
     """(infidelity, max-abs difference) between one column of each state (FP64 on device)."""
     out = np.zeros(5, dtype=np.float64)

@@ -274,3 +274,14 @@ def test_plan_search_mode_is_checked():
     g = GpuVectorState(12, 1, "c128")
     with pytest.raises(Exception):
         g.set_plan_search(2)
+
+
+def test_copy_bandwidth_is_plausible():
+    """The measured device copy (the streaming ceiling a sweep is compared
+    with) lands between 10 % and 100 % of the 8 TB/s HBM peak; bad sizes are
+    rejected."""
+    from q1tsim_amd.gvs import copy_bandwidth
+    ms, gbps = copy_bandwidth(1 << 30, 3)
+    assert ms > 0 and 800.0 < gbps < 8000.0, (ms, gbps)
+    with pytest.raises(Exception):
+        copy_bandwidth(0, 1)
