@@ -659,6 +659,18 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
                 A[(d + r) * d2 + c] = float(u.imag());
                 A[(d + r) * d2 + d + c] = float(u.real());
             }
+        if (nb <= 4) {
+            // MFMA fragment order for blocks of 3 and 4 qubits: lane (li, lg)
+            // of row tile rt reads A[16 rt + li][4 s + lg], s = 0 .. NS-1, as
+            // one contiguous run (vector loads instead of NS scattered ones)
+            const size_t NS = d2 / 4, NRT = d2 / 16;
+            std::vector<float> f(d2 * d2);
+            for (size_t rt = 0; rt < NRT; ++rt)
+                for (size_t lane = 0; lane < 64; ++lane)
+                    for (size_t s = 0; s < NS; ++s)
+                        f[(rt * 64 + lane) * NS + s] = A[(16 * rt + (lane & 15)) * d2 + 4 * s + (lane >> 4)];
+            std::copy(f.begin(), f.end(), A);
+        }
         uint32_t regs = 0;
         for (int lb : B) regs |= 1u << lb;
         SwPhase ph = make_phase(1, regs, M, off);

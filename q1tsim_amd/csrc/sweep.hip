@@ -292,14 +292,23 @@ template <int K, int M, int SW>
 __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict__ A, const SwPhase& ph, int tid)
 {
     constexpr int D = 1 << K, D2 = 2 * D, C = 1 << (M - K);
-    constexpr int NS = D2 / 4, NRT = D2 / 16;
+    constexpr int NS = D2 / 4, NRT = D2 / 16, NH = NS / 2;
     const int lane = tid & 63, wave = tid >> 6, nwaves = int(blockDim.x) >> 6;
     const int li = lane & 15, lg = lane >> 4;
+    // A fragments: the host stores blocks of 3 and 4 qubits in fragment
+    // order, lane's NS values of row tile rt contiguous (16-byte loads)
     float a[NRT][NS];
+    const float4* Af = reinterpret_cast<const float4*>(A);
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-        for (int s = 0; s < NS; ++s) a[rt][s] = A[(16 * rt + li) * D2 + 4 * s + lg];
+        for (int q = 0; q < NS / 4; ++q) {
+            const float4 t = Af[(rt * 64 + lane) * (NS / 4) + q];
+            a[rt][4 * q] = t.x;
+            a[rt][4 * q + 1] = t.y;
+            a[rt][4 * q + 2] = t.z;
+            a[rt][4 * q + 3] = t.w;
+        }
     uint32_t rbit[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) rbit[j] = 1u << ph.rpos[j];
@@ -311,34 +320,73 @@ __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict_
             if ((r >> (K - 1 - j)) & 1) p |= rbit[j];
         return p;
     };
-    for (int chunk = wave; chunk < C / 16; chunk += nwaves) {
-        const int c = chunk * 16 + li;
-        uint32_t cp = 0;
+    auto colpart = [&](int c) {
+        uint32_t p = 0;
 #pragma unroll
         for (int b = 0; b < M - K; ++b)
-            if ((c >> b) & 1) cp |= 1u << ph.tbit[b];
-        float bv[NS];
+            if ((c >> b) & 1) p |= 1u << ph.tbit[b];
+        return p;
+    };
+    // B fragment of one 16-column chunk: MFMA step s reads block row 4s+lg;
+    // rows D.. are the imaginary parts of rows 0.., so step s and step s+NH
+    // share one complex amplitude (one LDS read for both)
+    auto load_b = [&](uint32_t cp, float (&bv)[NS]) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int rr = 4 * s + lg;
-            const float2 v = lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))];
-            bv[s] = rr < D ? v.x : v.y;
+        for (int s = 0; s < NH; ++s) {
+            const float2 v = lds[swz<SW, M>(int(rowpart(4 * s + lg) | cp))];
+            bv[s] = v.x;
+            bv[s + NH] = v.y;
         }
-        f32x4 acc[NRT];
+    };
+    auto store = [&](uint32_t cp, const f32x4 (&acc)[NRT]) {
+        if constexpr (NRT == 2) {
+            // rows 4lg+v (real) and 16+4lg+v (imaginary) of one amplitude
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                lds[swz<SW, M>(int(rowpart(4 * lg + v) | cp))] = make_float2(acc[0][v], acc[1][v]);
+        } else {
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = 16 * rt + 4 * lg + v;
+                    float* dst = reinterpret_cast<float*>(&lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))]);
+                    dst[rr < D ? 0 : 1] = acc[rt][v];
+                }
+        }
+    };
+    // two chunks per pass: 2 NRT independent accumulator chains
+    for (int chunk = wave; chunk < C / 16; chunk += 2 * nwaves) {
+        const int chunk2 = chunk + nwaves;
+        const bool two = chunk2 < C / 16;
+        const uint32_t cp0 = colpart(chunk * 16 + li), cp1 = colpart(chunk2 * 16 + li);
+        float b0[NS], b1[NS];
+        load_b(cp0, b0);
+        if (two) load_b(cp1, b1);
+        f32x4 acc0[NRT], acc1[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt) {
-            acc[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], bv[s], acc[rt], 0, 0, 0);
+            acc0[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acc1[rt] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
+        if (two) {
 #pragma unroll
-        for (int rt = 0; rt < NRT; ++rt)
+            for (int s = 0; s < NS; ++s)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int rr = 16 * rt + 4 * lg + v;
-                float* dst = reinterpret_cast<float*>(&lds[swz<SW, M>(int(rowpart(rr & (D - 1)) | cp))]);
-                dst[rr < D ? 0 : 1] = acc[rt][v];
-            }
+                for (int rt = 0; rt < NRT; ++rt) {
+                    acc0[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b0[s], acc0[rt], 0, 0, 0);
+                    acc1[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b1[s], acc1[rt], 0, 0, 0);
+                }
+            store(cp0, acc0);
+            store(cp1, acc1);
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt)
+                    acc0[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][s], b0[s], acc0[rt], 0, 0, 0);
+            store(cp0, acc0);
+        }
     }
 }
 
