@@ -172,13 +172,14 @@ def train_cost_model(record_n=28, seed=5):
     itself: seeded exploration of the candidate programs (per-gate register
     phases, 3- and 4-qubit blocks; both tilings) on complex64 states of
     `record_n` qubits (2 GiB: tiles reach the high index bits the 30- and
-    32-qubit states use) running random-pair (depth 40) and layered (depth
-    20) circuits; minibatch Adam in the HIP kernels."""
+    32-qubit states use) running random-pair (depth 40), layered (depth 20)
+    and QFT circuits; minibatch Adam in the HIP kernels."""
     from q1tsim_amd import costmodel, workloads
     from q1tsim_amd.costmodel import CostModel
     t0 = time.perf_counter()
     circs = [workloads.random_pairs(record_n, 40, 1000 + k) for k in range(20)]
     circs += [workloads.random_layered(record_n, 20, 2000 + k) for k in range(10)]
+    circs += [workloads.qft(record_n)] * 8   # explored anew on every flush
     feat, tiles, ms = costmodel.record(record_n, circs, fusion_k=4, seed=seed)
     rec_s = time.perf_counter() - t0
     cm = CostModel(seed)

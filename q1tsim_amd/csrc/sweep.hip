@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <exception>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstring>
@@ -711,11 +713,11 @@ static double single_gate_us(double state_bytes) { return 2.0 * state_bytes / 4.
 // Pick the steps and sweep programs of a flush.  Plain (complex128, or
 // complex64 without fusion): per-gate register phases on the greedy tiles.
 // Fused complex64: greedy fixed-width blocks (PLAN_GREEDY), or the cheaper of
-// two tilings -- the per-gate tiling with register programs, and the fusion
-// tiling with the cheapest of {register phases, 3-qubit blocks, 4-qubit
-// blocks, blocks of the configured width when it is 5 or 6} per sweep -- as
-// ranked by the learned cost model (PLAN_LEARNED), or
-// a seeded random choice among the same candidates (PLAN_EXPLORE).
+// two tilings -- the per-gate tiling and the fusion tiling, each sweep of
+// either with the cheapest of {register phases, 3-qubit blocks, 4-qubit
+// blocks, blocks of the configured width when it is 5 or 6} -- as ranked by
+// the learned cost model (PLAN_LEARNED), or a seeded random choice among the
+// same candidates (PLAN_EXPLORE).
 static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queue, int fusion_k, double state_bytes,
                         double tiles, std::vector<PlanStep>& steps, std::vector<SweepProgram>& progs)
 {
@@ -744,72 +746,124 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
         progs = build_all(steps, Kg);
         return;
     }
-    // candidates: tiling A (per-gate) with register programs; tiling B
-    // (fusion) with one program per block width (0, 3, 4 and, above 4, fusion_k) per sweep
-    std::vector<PlanStep> sa = plan_gates(queue, n, g, 0, search), sb = plan_gates(queue, n, g, Kg, search);
-    std::vector<SweepProgram> pa = build_all(sa, 0);
+    // candidates: tiling A (per-gate tiles) and tiling B (fusion tiles), each
+    // sweep of either with one program per block width (0: register phases,
+    // 3, 4 and, above 4, fusion_k; in tiling A gates whose controls lie
+    // outside the tile stay register ops beside the blocks)
+    // (host planning runs on parallel threads: both tilings, then every
+    // candidate program; a flush's planning would otherwise rival its kernels
+    // on states of a few GiB)
+    std::vector<PlanStep> tl[2];
+    {
+        std::exception_ptr e0, e1;
+        std::thread other([&] {
+            try {
+                tl[1] = plan_gates(queue, n, g, Kg, search);
+            } catch (...) {
+                e1 = std::current_exception();
+            }
+        });
+        try {
+            tl[0] = plan_gates(queue, n, g, 0, search);
+        } catch (...) {
+            e0 = std::current_exception();
+        }
+        other.join();
+        if (e0) std::rethrow_exception(e0);
+        if (e1) std::rethrow_exception(e1);
+    }
     std::vector<int> kinds = {0, 3, 4};
     if (Kg > 4) kinds.push_back(Kg);
     const int nk = int(kinds.size());
-    std::vector<std::vector<SweepProgram>> pb(sb.size(), std::vector<SweepProgram>(kinds.size()));
-    for (size_t s = 0; s < sb.size(); ++s)
-        if (sb[s].sweep)
-            for (int c = 0; c < nk; ++c) pb[s][c] = build_candidate(queue, sb[s], g, kinds[c], search);
-    bool use_a = false;
-    std::vector<int> pick(sb.size(), 0);
+    std::vector<std::vector<SweepProgram>> cand[2];
+    std::vector<int> pick[2];
+    std::vector<std::array<size_t, 3>> work;   // (tiling, sweep, candidate)
+    for (int t = 0; t < 2; ++t) {
+        cand[t].assign(tl[t].size(), std::vector<SweepProgram>(kinds.size()));
+        pick[t].assign(tl[t].size(), 0);
+        for (size_t s = 0; s < tl[t].size(); ++s)
+            if (tl[t][s].sweep)
+                for (int c = 0; c < nk; ++c) work.push_back({size_t(t), s, size_t(c)});
+    }
+    {
+        std::atomic<size_t> next{0};
+        std::exception_ptr err;
+        std::mutex err_mu;
+        auto run = [&] {
+            try {
+                for (size_t i = next++; i < work.size(); i = next++) {
+                    const auto& w = work[i];
+                    cand[w[0]][w[1]][w[2]] = build_candidate(queue, tl[w[0]][w[1]], g, kinds[w[2]], search);
+                }
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (!err) err = std::current_exception();
+            }
+        };
+        const size_t hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+        const size_t nth = std::min(hw, std::max<size_t>(1, work.size() / 4));
+        std::vector<std::thread> pool;
+        for (size_t i = 1; i < nth; ++i) pool.emplace_back(run);
+        run();
+        for (auto& th : pool) th.join();
+        if (err) std::rethrow_exception(err);
+    }
+    int use = 1;
     if (mode == PLAN_EXPLORE) {
         Xoshiro256& rng = st.plan_rng();
-        use_a = (rng.next() >> 63) != 0;
-        for (size_t s = 0; s < sb.size(); ++s) pick[s] = int(rng.next() % uint64_t(nk));
+        use = int(rng.next() >> 63);
+        for (int t = 0; t < 2; ++t)
+            for (size_t s = 0; s < tl[t].size(); ++s) pick[t][s] = int(rng.next() % uint64_t(nk));
     } else {
         std::vector<float> feat;
-        std::vector<std::pair<int, int>> who;   // (step of A or -1-step of B, candidate)
+        struct Who {
+            int t;
+            size_t s;
+            int c;
+        };
+        std::vector<Who> who;
         float f[kSweepFeatures];
-        for (size_t s = 0; s < sa.size(); ++s)
-            if (sa[s].sweep) {
-                sweep_features(pa[s].phases, pa[s].ops, sa[s], prec, g, f);
-                feat.insert(feat.end(), f, f + kSweepFeatures);
-                who.emplace_back(int(s), 0);
-            }
-        for (size_t s = 0; s < sb.size(); ++s)
-            if (sb[s].sweep)
-                for (int c = 0; c < nk; ++c) {
-                    sweep_features(pb[s][c].phases, pb[s][c].ops, sb[s], prec, g, f);
-                    feat.insert(feat.end(), f, f + kSweepFeatures);
-                    who.emplace_back(-1 - int(s), c);
-                }
+        for (int t = 0; t < 2; ++t)
+            for (size_t s = 0; s < tl[t].size(); ++s)
+                if (tl[t][s].sweep)
+                    for (int c = 0; c < nk; ++c) {
+                        sweep_features(cand[t][s][c].phases, cand[t][s][c].ops, tl[t][s], prec, g, f);
+                        feat.insert(feat.end(), f, f + kSweepFeatures);
+                        who.push_back(Who{t, s, c});
+                    }
         std::vector<float> y(who.size());
         cm->predict(feat.data(), who.size(), y.data());
         const double us_per_unit = tiles / 1024.0;   // the model predicts log(us per 1024 tiles)
-        double cost_a = 0.0, cost_b = 0.0;
-        std::vector<double> best(sb.size(), 1e300);
-        for (size_t i = 0; i < who.size(); ++i) {
-            const double us = std::exp(double(y[i])) * us_per_unit;
-            if (who[i].first >= 0) {
-                cost_a += us;
-            } else {
-                const size_t s = size_t(-1 - who[i].first);
-                if (us < best[s]) {
-                    best[s] = us;
-                    pick[s] = who[i].second;
+        std::vector<std::vector<double>> pred[2];
+        for (int t = 0; t < 2; ++t) pred[t].assign(tl[t].size(), std::vector<double>(nk, 0.0));
+        for (size_t i = 0; i < who.size(); ++i)
+            pred[who[i].t][who[i].s][who[i].c] = std::exp(double(y[i])) * us_per_unit;
+        // per sweep: blocks replace the register program only when predicted at
+        // least 10 % cheaper (the fit's error is ~7 % in time, and a wrong pick
+        // of blocks costs more than a missed one); a tiling costs the sum of
+        // its chosen programs' predictions
+        constexpr double kBlockMargin = 0.9;
+        double cost[2] = {0.0, 0.0};
+        for (int t = 0; t < 2; ++t)
+            for (size_t s = 0; s < tl[t].size(); ++s) {
+                if (!tl[t][s].sweep) {
+                    cost[t] += single_gate_us(state_bytes) * double(tl[t][s].gates.size());
+                    continue;
                 }
+                int c_best = 0;
+                for (int c = 1; c < nk; ++c)
+                    if (pred[t][s][c] < kBlockMargin * pred[t][s][c_best] ||
+                        (c_best != 0 && pred[t][s][c] < pred[t][s][c_best]))
+                        c_best = c;
+                pick[t][s] = c_best;
+                cost[t] += pred[t][s][c_best];
             }
-        }
-        for (size_t s = 0; s < sa.size(); ++s)
-            if (!sa[s].sweep) cost_a += single_gate_us(state_bytes) * double(sa[s].gates.size());
-        for (size_t s = 0; s < sb.size(); ++s)
-            cost_b += sb[s].sweep ? best[s] : single_gate_us(state_bytes) * double(sb[s].gates.size());
-        use_a = cost_a < cost_b;
+        use = cost[0] < cost[1] ? 0 : 1;
     }
-    if (use_a) {
-        steps = std::move(sa);
-        progs = std::move(pa);
-    } else {
-        steps = std::move(sb);
-        progs.resize(steps.size());
-        for (size_t s = 0; s < steps.size(); ++s)
-            if (steps[s].sweep) progs[s] = std::move(pb[s][pick[s]]);
-    }
+    steps = std::move(tl[use]);
+    progs.resize(steps.size());
+    for (size_t s = 0; s < steps.size(); ++s)
+        if (steps[s].sweep) progs[s] = std::move(cand[use][s][pick[use][s]]);
 }
 
 void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size_t col_begin, size_t col_end,

@@ -120,11 +120,17 @@ def test_planner_modes_keep_amplitudes(trained, mode):
     assert infid <= 1e-6 and maxabs <= 1e-5, (mode, infid, maxabs)
 
 
-def test_learned_plan_vs_greedy_walltime(trained):
-    """Wall time of the learned plan against greedy fixed-width fusion on
-    held-out circuits (median of 3 flushes each); written to gpurun_out."""
-    cm, loss = trained
-    n = 24
+def test_learned_plan_vs_greedy_walltime():
+    """What the trained network gains over the method without it: wall time
+    (host planning included, median of 3 flushes) of the learned plan against
+    greedy fixed-width fusion on a held-out 28-qubit random-pair circuit,
+    with a model recorded on 26-qubit states; written to gpurun_out."""
+    n, rec_n = 28, 26
+    circs = [workloads.random_pairs(rec_n, 40, 500 + k) for k in range(8)]
+    circs += [workloads.random_layered(rec_n, 12, 600 + k) for k in range(4)]
+    feat, tiles, ms = costmodel.record(rec_n, circs, fusion_k=4, seed=3)
+    cm = CostModel(5)
+    loss = costmodel.fit(cm, feat, tiles, ms, epochs=40)
     ops = workloads.random_pairs(n, 40, 12345)
     res = {}
     for mode in (0, 1):
@@ -142,8 +148,12 @@ def test_learned_plan_vs_greedy_walltime(trained):
         s.close()
     greedy, learned = res[0][0], res[1][0]
     os.makedirs("gpurun_out", exist_ok=True)
-    with open("gpurun_out/learned_vs_greedy_n24.json", "w") as f:
-        json.dump({"n": n, "train_mse": loss, "greedy_s": greedy, "learned_s": learned,
-                   "greedy_stats": res[0][1], "learned_stats": res[1][1]}, f)
+    with open("gpurun_out/learned_vs_greedy_n%d.json" % n, "w") as f:
+        json.dump({"n": n, "record_n": rec_n, "samples": int(len(feat)), "train_mse": loss, "greedy_s": greedy,
+                   "learned_s": learned, "greedy_stats": res[0][1], "learned_stats": res[1][1]}, f)
     print("learned/greedy wall time: %.4f / %.4f s" % (learned, greedy))
-    assert learned <= 1.25 * greedy
+    # the learned plan moves less data (fewer sweeps) ...
+    assert res[1][1]["sweeps"] <= res[0][1]["sweeps"]
+    # ... and is not slower beyond timing noise (5 %); its gain at 32 qubits,
+    # where the kernels dwarf planning, is in the bench line
+    assert learned <= 1.05 * greedy
