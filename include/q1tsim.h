@@ -85,6 +85,13 @@ int q1t_circuit_read_amplitudes(circuit_t* ptr, size_t col, double* out, size_t 
 int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out);
 /* histogram_vec (circuit.rs:800-815): counts of length 2^nr_cbits */
 int q1t_circuit_histogram_vec(const circuit_t* ptr, size_t* out, size_t len);
+/* The order in which the plain (per-gate) GPU plan of this gate-only circuit
+ * applies its gates (indices in add order; identity gates never run and are
+ * left out), for the default tile shape of `precision`; search != 0 refines
+ * tiles and register phases by local search.  A planning report, computed on
+ * the host: it needs no GPU. */
+int q1t_circuit_plan_order(const circuit_t* ptr, int precision, int search, size_t* out, size_t cap,
+                           size_t* count);
 
 /* ---- batches of circuits (no ffi.rs counterpart: the reference runs one
  * circuit per Circuit::execute, circuit.rs:562-600) ----

@@ -90,6 +90,7 @@ SIGNATURES = {
     "q1t_circuit_read_amplitudes": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_circuit_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_circuit_histogram_vec": (_i32, [_vp, _ps, _sz]),
+    "q1t_circuit_plan_order": (_i32, [_vp, _i32, _i32, _ps, _sz, _ps]),
     "q1t_cm_new": (_i32, [_u64, ctypes.POINTER(_vp)]),
     "q1t_cm_free": (None, [_vp]),
     "q1t_cm_get_params": (_i32, [_vp, _pf, _sz]),

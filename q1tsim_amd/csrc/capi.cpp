@@ -291,6 +291,30 @@ extern "C" int q1t_circuit_barrier(circuit_t* ptr, const size_t* qbits, size_t n
     return guard([&] { ptr->c.barrier(qbits, n); });
 }
 
+extern "C" int q1t_circuit_plan_order(const circuit_t* ptr, int precision, int search, size_t* out, size_t cap,
+                                      size_t* count)
+{
+    NEED(ptr);
+    NEED(count);
+    return guard([&] {
+        if (precision != PREC_C128 && precision != PREC_C64) throw Error(ERR_INVALID_ARGUMENT, "unknown precision");
+        const size_t n = ptr->c.nr_qbits();
+        std::vector<PendingGate> q;
+        for (const CircuitOp& op : ptr->c.ops()) {
+            if (op.type != OpType::Gate) throw Error(ERR_INVALID_ARGUMENT, "plan order: the circuit holds non-gate ops");
+            std::vector<double> pv;
+            for (const Param& p : op.params) pv.push_back(p.get());
+            const std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+            PendingGate pg;
+            pg.op = analyze_gate(m.data(), op.bits.size(), op.bits.data(), n);
+            q.push_back(std::move(pg));
+        }
+        const std::vector<size_t> order = plan_order(q, int(n), tile_geometry(Precision(precision)), search != 0);
+        *count = order.size();
+        for (size_t i = 0; i < order.size() && i < cap && out; ++i) out[i] = order[i];
+    });
+}
+
 extern "C" int q1t_circuit_nr_columns(circuit_t* ptr, size_t* out)
 {
     NEED(ptr);

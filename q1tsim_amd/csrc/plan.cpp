@@ -431,7 +431,7 @@ size_t phase_absorb(const std::vector<PendingGate>& q, const std::vector<size_t>
 // more gates: every phase costs a full LDS round trip of the tile.
 std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::vector<size_t>& pending,
                                   const LocalMap& lm, int M, int R, std::vector<SwPhase>& phases,
-                                  std::vector<SwOp>& ops, bool search)
+                                  std::vector<SwOp>& ops, bool search, std::vector<size_t>* order)
 {
     uint32_t regs = 0;
     int nregs = 0;
@@ -484,6 +484,12 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
     // the write-back map, composed in queue order: f(l) = XOR_j l_j col[j] ^ c
     uint32_t col[16], cst = 0;
     for (int j = 0; j < 16; ++j) col[j] = 1u << j;
+    if (order) {   // execution order: the register ops, then the write-back gates
+        for (size_t i : chosen)
+            if (phase_role(q[i].op, lm, regs) == ROLE_OP) order->push_back(i);
+        for (size_t i : chosen)
+            if (phase_role(q[i].op, lm, regs) != ROLE_OP) order->push_back(i);
+    }
     auto apply = [&](auto&& g) {   // f := g o f for a map g on local indices
         for (int j = 0; j < M; ++j) col[j] = g(col[j], false);
         cst = g(cst, true);
@@ -590,11 +596,26 @@ void compose(std::vector<cdouble>& U, int nb, const GateOp& op, const int* pos)
 }  // namespace
 
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search)
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search, std::vector<size_t>* order)
 {
     const LocalMap lm(step.tile);
     std::vector<size_t> pending = step.gates;
-    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search);
+    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, order);
+}
+
+std::vector<size_t> plan_order(const std::vector<PendingGate>& q, int n, const TileGeom& g, bool search)
+{
+    std::vector<size_t> order;
+    for (const PlanStep& st : plan_gates(q, n, g, 0, search)) {
+        if (!st.sweep) {
+            order.insert(order.end(), st.gates.begin(), st.gates.end());
+            continue;
+        }
+        std::vector<SwPhase> phases;
+        std::vector<SwOp> ops;
+        build_program(q, st, g.M, g.R, phases, ops, search, &order);
+    }
+    return order;
 }
 
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
@@ -632,7 +653,7 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
             absorbed.push_back(i);
         }
         if (absorbed.empty()) {
-            pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search);
+            pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, nullptr);
             continue;
         }
         // pad to at least 3 bits (the MFMA tile needs 2D >= 16)

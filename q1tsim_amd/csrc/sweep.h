@@ -75,9 +75,16 @@ bool sweepable(const GateOp& op, std::vector<int>* local_targets);
 std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k,
                                  bool search = true);
 
-// Register phases of one sweep (per-gate arithmetic).
+// Register phases of one sweep (per-gate arithmetic).  `order`, if given,
+// receives the queue indices of the sweep's gates in the order the kernel
+// applies them (per phase: its register ops, then its write-back gates).
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search = true);
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search = true,
+                   std::vector<size_t>* order = nullptr);
+
+// The order in which the plain (per-gate) plan applies the queue's gates
+// (identity gates, which never run, are left out).
+std::vector<size_t> plan_order(const std::vector<PendingGate>& q, int n, const TileGeom& g, bool search);
 
 // Fused program of one sweep: dense blocks of at most K local qubits
 // (composed in FP64, real-embedded, rounded to FP32 into `blocks`) and

@@ -306,6 +306,17 @@ class Circuit(object):
         _lib.check(self.__sim.q1t_circuit_histogram_vec(self.__ptr, out, n))
         return [int(out[i]) for i in range(n)]
 
+    def plan_order(self, precision="c128", search=True):
+        """Gate indices (in add order) in the order the plain GPU plan applies
+        them; computed on the host, no GPU needed (gate-only circuits)."""
+        prec = {"c128": 0, "c64": 1}[precision]
+        cnt = ctypes.c_size_t(0)
+        _lib.check(self.__sim.q1t_circuit_plan_order(self.__ptr, prec, 1 if search else 0, None, 0, ctypes.byref(cnt)))
+        out = (ctypes.c_size_t * max(1, cnt.value))()
+        _lib.check(self.__sim.q1t_circuit_plan_order(self.__ptr, prec, 1 if search else 0, out, cnt.value,
+                                                     ctypes.byref(cnt)))
+        return [int(out[i]) for i in range(cnt.value)]
+
     @property
     def handle(self):
         """The circuit_t* of this circuit (for batches)."""
