@@ -680,10 +680,10 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
                 A[(d + r) * d2 + c] = float(u.imag());
                 A[(d + r) * d2 + d + c] = float(u.real());
             }
-        if (nb <= 4) {
-            // MFMA fragment order for blocks of 3 and 4 qubits: lane (li, lg)
-            // of row tile rt reads A[16 rt + li][4 s + lg], s = 0 .. NS-1, as
-            // one contiguous run (vector loads instead of NS scattered ones)
+        {
+            // MFMA fragment order: lane (li, lg) of row tile rt reads
+            // A[16 rt + li][4 s + lg], s = 0 .. NS-1, as one contiguous run
+            // (vector loads instead of NS scattered ones)
             const size_t NS = d2 / 4, NRT = d2 / 16;
             std::vector<float> f(d2 * d2);
             for (size_t rt = 0; rt < NRT; ++rt)

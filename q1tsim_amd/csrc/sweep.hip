@@ -393,9 +393,10 @@ __device__ __forceinline__ void block_mfma(float2* lds, const float* __restrict_
 }
 
 // The same block product for K = 5 or 6 (2D = 64 or 128 rows): the 2D x 2D
-// real matrix (16 or 64 KiB) is too large to keep per lane, so each MFMA
-// reads its A fragment from global memory (L1/L2-resident: every tile of the
-// sweep uses the same matrix); B fragments and accumulators stay in registers.
+// real matrix (16 or 64 KiB) is too large to keep per lane, so each slice of
+// 8 MFMA steps reads its A values from global memory (L1/L2-resident: every
+// tile of the sweep uses the same matrix; fragment order makes them two
+// 16-byte loads per row tile); B fragments and accumulators stay in registers.
 template <int K, int M, int SW>
 __device__ __forceinline__ void block_mfma_wide(float2* lds, const float* __restrict__ A, const SwPhase& ph, int tid)
 {
@@ -436,10 +437,13 @@ __device__ __forceinline__ void block_mfma_wide(float2* lds, const float* __rest
             }
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) {
-                const float* arow = A + (16 * rt + li) * D2 + 4 * s0 + lg;
+                // fragment order: this lane's 8 values of the slice are contiguous
+                const float4* af = reinterpret_cast<const float4*>(A + (rt * 64 + lane) * NS + s0);
+                const float4 f0 = af[0], f1 = af[1];
+                const float as[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
 #pragma unroll
                 for (int s = 0; s < 8; ++s)
-                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * s], bv[s], acc[rt], 0, 0, 0);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[s], bv[s], acc[rt], 0, 0, 0);
             }
         }
 #pragma unroll
