@@ -656,6 +656,47 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
             pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, nullptr);
             continue;
         }
+        if (search && int(B.size()) == K) {
+            // local search over the block's K local bits: exchange one for
+            // another while the block absorbs more gates (fewer blocks)
+            auto absorb_set = [&](uint32_t S, std::vector<size_t>* ab, std::vector<size_t>* rs) {
+                Blocker bk;
+                size_t cnt = 0;
+                for (size_t i : pending) {
+                    const GateOp& op = q[i].op;
+                    const uint64_t bits = touched_bits(op), nd = nondiag_bits(op);
+                    bool ok = !bk.blocks(bits, nd);
+                    for (int b = 0; b < 64 && ok; ++b)
+                        if ((bits >> b) & 1) {
+                            const int lb = lm.local_of[b];
+                            ok = lb >= 0 && ((S >> lb) & 1);
+                        }
+                    if (ok) { ++cnt; if (ab) ab->push_back(i); }
+                    else { bk.add(bits, nd); if (rs) rs->push_back(i); }
+                }
+                return cnt;
+            };
+            uint32_t S = 0;
+            for (int lb : B) S |= 1u << lb;
+            size_t best = absorb_set(S, nullptr, nullptr);
+            for (int iter = 0; iter < 4 * M; ++iter) {
+                bool improved = false;
+                for (int a = 0; a < M && !improved; ++a) {
+                    if (!((S >> a) & 1)) continue;
+                    for (int b = 0; b < M; ++b) {
+                        if ((S >> b) & 1) continue;
+                        const uint32_t S2 = (S & ~(1u << a)) | (1u << b);
+                        const size_t c = absorb_set(S2, nullptr, nullptr);
+                        if (c > best) { best = c; S = S2; improved = true; break; }
+                    }
+                }
+                if (!improved) break;
+            }
+            absorbed.clear(); rest.clear();
+            absorb_set(S, &absorbed, &rest);
+            B.clear();
+            for (int b = 0; b < M; ++b) if ((S >> b) & 1) B.push_back(b);
+        }
         // pad to at least 3 bits (the MFMA tile needs 2D >= 16)
         for (int b = 0; b < M && B.size() < 3; ++b)
             if (std::find(B.begin(), B.end(), b) == B.end()) B.push_back(b);
