@@ -227,9 +227,14 @@ template <> __device__ __forceinline__ double leaf_value<double2>(const double2*
 template <> __device__ __forceinline__ double leaf_value<float2>(const float2* in, uint64_t k) { return weight(in[k]); }
 template <> __device__ __forceinline__ double leaf_value<double>(const double* in, uint64_t k) { return in[k]; }
 
+// One tree level of every column at once: column blockIdx.y reads at
+// in + y * in_stride and writes at out + y * out_stride.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_tree_level(const T* __restrict__ in, uint64_t size, double* __restrict__ out)
+__global__ void __launch_bounds__(kBlock) k_tree_level(const T* __restrict__ in, uint64_t in_stride, uint64_t size,
+                                                       double* __restrict__ out, uint64_t out_stride)
 {
+    in += uint64_t(blockIdx.y) * in_stride;
+    out += uint64_t(blockIdx.y) * out_stride;
     __shared__ double w[kTreeFanout * (kBlock + 1)];
     const uint64_t nout = (size + kTreeFanout - 1) / kTreeFanout;
     const uint64_t chunk = uint64_t(kBlock) * kTreeFanout;
@@ -259,25 +264,38 @@ struct TreeDesc {
     const double* lev[16];   // lev[0] unused: level 0 is read from the amplitudes
 };
 
+// Entry k of level l of the column whose tree levels start `coff` entries
+// after column 0's (level 0 is that column's amplitudes `a`).
 template <typename V>
-__device__ __forceinline__ double tree_value(const V* a, const TreeDesc& td, int l, uint64_t k)
+__device__ __forceinline__ double tree_value(const V* a, const TreeDesc& td, int l, uint64_t k, uint64_t coff)
 {
-    return l == 0 ? weight(a[k]) : td.lev[l][k];
+    return l == 0 ? weight(a[k]) : td.lev[l][coff + k];
 }
 
 // Inverse-CDF sampling of one run per thread by the canonical tree walk:
 // t = u * T; within each group keep offset += S_j while offset + S_j <= t,
 // descend into the first child where that fails, or the last positive one.
+// All runs of all columns in one launch: run r belongs to the last column c
+// with first_run[c] <= r (columns in order; first_run[0] = 0).
 template <typename V>
-__global__ void __launch_bounds__(kBlock) k_tree_sample(const V* __restrict__ a, TreeDesc td,
-                                                        const double* __restrict__ u01, uint64_t nruns,
+__global__ void __launch_bounds__(kBlock) k_tree_sample(const V* __restrict__ a0, uint64_t col_stride, TreeDesc td,
+                                                        uint64_t per_col, const uint64_t* __restrict__ first_run,
+                                                        int ncols, const double* __restrict__ u01, uint64_t nruns,
                                                         uint64_t* __restrict__ out)
 {
     const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= nruns) return;
+    int lo = 0, hi = ncols - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first_run[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    const V* a = a0 + uint64_t(lo) * col_stride;
+    const uint64_t coff = uint64_t(lo) * per_col;
     const int top = td.nlev - 1;
-    double total = tree_value(a, td, top, 0);
-    for (uint64_t k = 1; k < td.size[top]; ++k) total += tree_value(a, td, top, k);
+    double total = tree_value(a, td, top, 0, coff);
+    for (uint64_t k = 1; k < td.size[top]; ++k) total += tree_value(a, td, top, k, coff);
     const double t = u01[r] * total;
     double offset = 0.0;
     uint64_t gstart = 0, glen = td.size[top], idx = 0;
@@ -286,7 +304,7 @@ __global__ void __launch_bounds__(kBlock) k_tree_sample(const V* __restrict__ a,
         int last = 0;
 #pragma unroll
         for (int j = 0; j < kTreeFanout; ++j) {
-            v[j] = (uint64_t(j) < glen) ? tree_value(a, td, l, gstart + j) : 0.0;
+            v[j] = (uint64_t(j) < glen) ? tree_value(a, td, l, gstart + j, coff) : 0.0;
             if (v[j] > 0.0) last = j;
         }
         int chosen = last;
@@ -1156,12 +1174,15 @@ static TreeDesc tree_shape(uint64_t dim, uint64_t* entries)
 
 size_t GpuVectorState::measure_all_host_bytes() const
 {
-    return nr_shots_ * (sizeof(double) + sizeof(uint64_t)) + counts_.size() * 32 * sizeof(double);
+    return nr_shots_ * (sizeof(double) + sizeof(uint64_t)) + counts_.size() * 32 * sizeof(double) +
+           (counts_.size() + 1) * sizeof(uint64_t);
 }
 
 // Host scratch of measure_all_async: [uniforms: nr_shots doubles][indices:
 // nr_shots u64][per column: 32 doubles holding the top tree level (or the
-// amplitudes of a state of at most 16 entries)].
+// amplitudes of a state of at most 16 entries)][first run of each column,
+// nc + 1 u64].  Every launch covers all columns: one per tree level, one 2-D
+// copy of the top levels, one sampling launch.
 void GpuVectorState::measure_all_async(void* host)
 {
     flush();
@@ -1170,8 +1191,9 @@ void GpuVectorState::measure_all_async(void* host)
     double* hu = static_cast<double*>(host);
     uint64_t* hidx = reinterpret_cast<uint64_t*>(hu + nr_shots_);
     double* htop = reinterpret_cast<double*>(hidx + nr_shots_);
+    uint64_t* hfirst = reinterpret_cast<uint64_t*>(htop + 32 * nc);
     uint64_t per_col = 0;
-    TreeDesc shape = tree_shape(dim(), &per_col);
+    TreeDesc td = tree_shape(dim(), &per_col);
     const size_t need = std::max<size_t>(per_col * nc, 1);
     if (need > tree_cap_) {
         dev_release(d_tree_, tree_cap_ * sizeof(double));
@@ -1180,59 +1202,71 @@ void GpuVectorState::measure_all_async(void* host)
         d_tree_ = static_cast<double*>(dev_alloc(need * sizeof(double)));
         tree_cap_ = need;
     }
-    const size_t ub = nr_shots_ * sizeof(double);
-    char* buf = static_cast<char*>(small_buf(2 * ub + 64));
-    double* du = reinterpret_cast<double*>(buf);
-    uint64_t* dout = reinterpret_cast<uint64_t*>(buf + ub);
-    HIP_OK(hipMemcpyAsync(du, hu, ub, hipMemcpyHostToDevice, stream_));
-    size_t run = 0;
-    for (size_t c = 0; c < nc; ++c) {
-        const size_t count = counts_[c];
-        if (count == 0) continue;
-        TreeDesc td = shape;
-        uint64_t o = per_col * c;
+    {   // column 0's level pointers; column c's are per_col entries further
+        uint64_t o = 0;
         for (int l = 1; l < td.nlev; ++l) {
             td.lev[l] = d_tree_ + o;
             o += td.size[l];
         }
-        const void* a = col_ptr(c);
+    }
+    uint64_t run = 0;
+    for (size_t c = 0; c < nc; ++c) {
+        hfirst[c] = run;
+        run += counts_[c];
+    }
+    hfirst[nc] = run;
+    const size_t ub = nr_shots_ * sizeof(double), fb = (nc + 1) * sizeof(uint64_t);
+    char* buf = static_cast<char*>(small_buf(2 * ub + fb + 64));
+    double* du = reinterpret_cast<double*>(buf);
+    uint64_t* dout = reinterpret_cast<uint64_t*>(buf + ub);
+    uint64_t* dfirst = reinterpret_cast<uint64_t*>(buf + 2 * ub);
+    HIP_OK(hipMemcpyAsync(du, hu, ub, hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipMemcpyAsync(dfirst, hfirst, fb, hipMemcpyHostToDevice, stream_));
+    const uint64_t cstride = dim();
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
         for (int l = 1; l < td.nlev; ++l) {
             const uint64_t chunks =
                 (td.size[l - 1] + uint64_t(kBlock) * kTreeFanout - 1) / (uint64_t(kBlock) * kTreeFanout);
-            const unsigned g = unsigned(std::min<uint64_t>(chunks, 65536));
-            double* outp = const_cast<double*>(td.lev[l]);
+            const dim3 grid(unsigned(std::min<uint64_t>(chunks, 65536)), ny);
+            double* outp = const_cast<double*>(td.lev[l]) + per_col * c0;
             if (l == 1) {
                 if (prec_ == PREC_C128)
-                    hipLaunchKernelGGL(k_tree_level<double2>, dim3(g), dim3(kBlock), 0, stream_,
-                                       static_cast<const double2*>(a), td.size[0], outp);
+                    hipLaunchKernelGGL(k_tree_level<double2>, grid, dim3(kBlock), 0, stream_,
+                                       static_cast<const double2*>(col_ptr(c0)), cstride, td.size[0], outp, per_col);
                 else
-                    hipLaunchKernelGGL(k_tree_level<float2>, dim3(g), dim3(kBlock), 0, stream_,
-                                       static_cast<const float2*>(a), td.size[0], outp);
+                    hipLaunchKernelGGL(k_tree_level<float2>, grid, dim3(kBlock), 0, stream_,
+                                       static_cast<const float2*>(col_ptr(c0)), cstride, td.size[0], outp, per_col);
             } else {
-                hipLaunchKernelGGL(k_tree_level<double>, dim3(g), dim3(kBlock), 0, stream_, td.lev[l - 1],
-                                   td.size[l - 1], outp);
+                hipLaunchKernelGGL(k_tree_level<double>, grid, dim3(kBlock), 0, stream_, td.lev[l - 1] + per_col * c0,
+                                   per_col, td.size[l - 1], outp, per_col);
             }
             HIP_OK(hipGetLastError());
         }
-        // the top level (or the amplitudes themselves) for the norm check
-        const int top = td.nlev - 1;
+    }
+    // the top levels (or the amplitudes themselves) for the norm check
+    const int top = td.nlev - 1;
+    if (nc) {
         if (top == 0)
-            HIP_OK(hipMemcpyAsync(htop + 32 * c, a, size_t(td.size[0]) * elem_bytes(), hipMemcpyDeviceToHost,
-                                  stream_));
+            HIP_OK(hipMemcpy2DAsync(htop, 32 * sizeof(double), col_ptr(0), size_t(cstride) * elem_bytes(),
+                                    size_t(td.size[0]) * elem_bytes(), nc, hipMemcpyDeviceToHost, stream_));
         else
-            HIP_OK(hipMemcpyAsync(htop + 32 * c, td.lev[top], size_t(td.size[top]) * sizeof(double),
-                                  hipMemcpyDeviceToHost, stream_));
-        // sampling is in bounds for any weights (an all-zero group walks to
-        // its first child); invalid norms are reported by measure_all_finish
-        const unsigned g = unsigned((count + kBlock - 1) / kBlock);
+            HIP_OK(hipMemcpy2DAsync(htop, 32 * sizeof(double), td.lev[top], size_t(per_col) * sizeof(double),
+                                    size_t(td.size[top]) * sizeof(double), nc, hipMemcpyDeviceToHost, stream_));
+    }
+    // sampling is in bounds for any weights (an all-zero group walks to its
+    // first child); invalid norms are reported by measure_all_finish
+    if (run) {
+        const unsigned g = unsigned((run + kBlock - 1) / kBlock);
         if (prec_ == PREC_C128)
             hipLaunchKernelGGL(k_tree_sample<double2>, dim3(g), dim3(kBlock), 0, stream_,
-                               static_cast<const double2*>(a), td, du + run, uint64_t(count), dout + run);
+                               static_cast<const double2*>(col_ptr(0)), cstride, td, per_col, dfirst, int(nc), du,
+                               run, dout);
         else
             hipLaunchKernelGGL(k_tree_sample<float2>, dim3(g), dim3(kBlock), 0, stream_,
-                               static_cast<const float2*>(a), td, du + run, uint64_t(count), dout + run);
+                               static_cast<const float2*>(col_ptr(0)), cstride, td, per_col, dfirst, int(nc), du,
+                               run, dout);
         HIP_OK(hipGetLastError());
-        run += count;
     }
     HIP_OK(hipMemcpyAsync(hidx, dout, nr_shots_ * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
 }
