@@ -203,12 +203,14 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             std::vector<Circuit*> still, meas;
             std::vector<size_t> off;
             std::vector<char> kind;   // 1: measure / peek, 2: measure_all / peek_all
+            std::vector<char> measured(active.size(), 0);
             size_t total = 0;
             for (size_t k = 0; k < active.size(); ++k) {
                 if (!more[k]) continue;
                 Circuit* c = active[k];
                 const char kd = c->next_is_z_measure() ? 1 : c->next_is_z_measure_all() ? 2 : 0;
                 if (!kd) continue;
+                measured[k] = 1;
                 meas.push_back(c);
                 kind.push_back(kd);
                 off.push_back(total);
@@ -231,7 +233,7 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             lap(stats_.measure_s);
             for (size_t k = 0; k < active.size(); ++k) {
                 if (more[k]) {
-                    if (std::find(meas.begin(), meas.end(), active[k]) == meas.end()) {
+                    if (!measured[k]) {
                         active[k]->run_next_op();
                         stats_.sync_ops += 1;
                     }

@@ -560,7 +560,9 @@ void GpuVectorState::set_geometry(const TileGeom& g)
 
 GpuVectorState::~GpuVectorState()
 {
-    if (stream_) (void)hipStreamSynchronize(stream_);
+    // an own stream is destroyed below, so its work must end first; a shared
+    // stream outlives the state, and the released blocks are event-guarded
+    if (stream_ && !shared_) (void)hipStreamSynchronize(stream_);
     try {
         DeviceCache& dc = DeviceCache::get();
         dc.release(amp_, amp_cols_ * dim() * elem_bytes(), stream_);
@@ -638,14 +640,12 @@ void GpuVectorState::set_zero_state()
     counts_.assign(1, nr_shots_);
     alloc_cols(1);
     HIP_OK(hipMemsetAsync(amp_, 0, dim() * elem_bytes(), stream_));
-    if (prec_ == PREC_C128) {
-        const double one[2] = {1.0, 0.0};
-        HIP_OK(hipMemcpyAsync(amp_, one, sizeof(one), hipMemcpyHostToDevice, stream_));
-    } else {
-        const float one[2] = {1.0f, 0.0f};
-        HIP_OK(hipMemcpyAsync(amp_, one, sizeof(one), hipMemcpyHostToDevice, stream_));
-    }
-    HIP_OK(hipStreamSynchronize(stream_));
+    // static sources: the copies may run after this returns (same stream
+    // order for everything that follows, no wait needed)
+    static const double one64[2] = {1.0, 0.0};
+    static const float one32[2] = {1.0f, 0.0f};
+    if (prec_ == PREC_C128) HIP_OK(hipMemcpyAsync(amp_, one64, sizeof(one64), hipMemcpyHostToDevice, stream_));
+    else HIP_OK(hipMemcpyAsync(amp_, one32, sizeof(one32), hipMemcpyHostToDevice, stream_));
 }
 
 void GpuVectorState::set_product_state(const cdouble* coefs)
