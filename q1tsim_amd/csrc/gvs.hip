@@ -684,15 +684,18 @@ void GpuVectorState::set_amplitudes(const cdouble* amps)
     counts_.assign(1, nr_shots_);
     alloc_cols(1);
     if (prec_ == PREC_C128) {
-        HIP_OK(hipMemcpy(amp_, amps, dim() * sizeof(cdouble), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpyAsync(amp_, amps, dim() * sizeof(cdouble), hipMemcpyHostToDevice, stream_));
     } else {
         std::vector<float> f(2 * dim());
         for (uint64_t i = 0; i < dim(); ++i) {
             f[2 * i] = float(amps[i].real());
             f[2 * i + 1] = float(amps[i].imag());
         }
-        HIP_OK(hipMemcpy(amp_, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpyAsync(amp_, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, stream_));
     }
+    // stream-ordered (earlier kernels may still use the buffer); the host
+    // data must outlive the copy
+    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void GpuVectorState::set_random_normal(uint64_t seed)

@@ -916,7 +916,10 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
                             progs[s].blocks.size() * sizeof(float));
         }
         dprog = static_cast<char*>(st.scratch(GpuVectorState::SLOT_PROGRAM, total));
-        SW_HIP_OK(hipMemcpy(dprog, host.data(), total, hipMemcpyHostToDevice));
+        // ordered after the previous flush's sweeps on this (non-blocking)
+        // stream, which may still read the table
+        SW_HIP_OK(hipMemcpyAsync(dprog, host.data(), total, hipMemcpyHostToDevice, st.stream()));
+        SW_HIP_OK(hipStreamSynchronize(st.stream()));
     }
     const float* dblocks = reinterpret_cast<const float*>(dprog + boff);
     for (size_t s = 0; s < steps.size(); ++s) {

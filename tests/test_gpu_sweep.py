@@ -252,6 +252,31 @@ def test_search_and_swaps_keep_per_gate_result():
     assert res[(1, True)][1]["sweeps"] <= res[(0, True)][1]["sweeps"]
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_back_to_back_flushes(prec):
+    """Flushes issued one after another with nothing read between them: each
+    flush's sweep program replaces the previous one's while those sweeps may
+    still be queued, so the upload must wait for them."""
+    n = 20
+    a = rand_state(n, 31)
+    segments = [mixed_ops(n, 12, 100 + s) for s in range(10)]
+    g = GpuVectorState.from_amplitudes(a, 1, prec)
+    g.set_plan(4 if prec == "c64" else 0, True)
+    for ops in segments:
+        for name, bits, params in ops:
+            g.apply_gate(orc.gate_matrix(name, params), bits)
+        g.flush()
+    ref = oracle(n, [op for ops in segments for op in ops], a)[:, 0]
+    col = g.states[:, 0]
+    err = float(np.abs(col - ref).max())
+    assert g.stats()["sweeps"] >= 10
+    if prec == "c128":
+        assert err <= 1e-12, err
+    else:
+        fid = abs(np.vdot(ref, col)) ** 2 / float(np.vdot(col, col).real)
+        assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+
+
 def test_fused_search_c64():
     n = 17
     a = rand_state(n, 9)
