@@ -132,6 +132,16 @@ int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling);
  * mode -1 (default) for states of 512 MiB and more, 0 never, 1 always.
  * Changes only the schedule, not the gates applied. */
 int q1t_gvs_set_plan_search(q1t_gvs* s, int mode);
+/* Build-added, no counterpart in the reference: flush graphs (on = 1).  The
+ * kernel launches of a planned flush are captured into one hipGraph; a later
+ * flush whose launch signature (tiles, program sizes, single-gate kernels
+ * with their matrices, columns, buffer addresses) is the same replays it
+ * after uploading its own sweep programs, so new gate parameters reach the
+ * replay.  Off while profiling or recording; a flush with a 3+-target
+ * single-gate kernel runs eagerly.  Results equal the eager path's. */
+int q1t_gvs_set_graph(q1t_gvs* s, int on);
+/* Build-added: graphs captured and replays launched over the state's life. */
+int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* replays);
 /* Build-added: tile geometry of the sweeps (M tile bits, R register bits,
  * L low bits per tile); only the compiled shapes are accepted
  * (complex128: M=12 with R=3 or 4; complex64: M=12 with R=3 or 4, M=13 with R=4). */

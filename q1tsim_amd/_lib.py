@@ -137,6 +137,8 @@ SIGNATURES = {
     "q1t_gvs_flush": (_i32, [_vp]),
     "q1t_gvs_set_plan": (_i32, [_vp, _i32, _i32]),
     "q1t_gvs_set_plan_search": (_i32, [_vp, _i32]),
+    "q1t_gvs_set_graph": (_i32, [_vp, _i32]),
+    "q1t_gvs_graph_stats": (_i32, [_vp, _pu64, _pu64]),
     "q1t_gvs_set_geometry": (_i32, [_vp, _i32, _i32, _i32]),
     "q1t_gvs_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),
     "q1t_gvs_set_profiling": (_i32, [_vp, _i32]),

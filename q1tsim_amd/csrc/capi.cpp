@@ -676,6 +676,25 @@ extern "C" int q1t_gvs_set_plan_search(q1t_gvs* s, int mode)
     });
 }
 
+extern "C" int q1t_gvs_set_graph(q1t_gvs* s, int on)
+{
+    NEED(s);
+    return guard([&] {
+        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "graph mode must be 0 or 1");
+        s->s.set_graph(on != 0);
+    });
+}
+
+extern "C" int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* replays)
+{
+    NEED(s);
+    NEED(captures);
+    NEED(replays);
+    *captures = s->s.flush_graph().captures;
+    *replays = s->s.flush_graph().replays;
+    return 0;
+}
+
 extern "C" int q1t_gvs_set_plan(q1t_gvs* s, int fusion_k, int tiling)
 {
     NEED(s);

@@ -208,6 +208,29 @@ public:
     std::vector<SweepSample> take_samples(size_t max);
     size_t sample_count() const { return samples_done_; }
 
+    // Flush graphs (off by default): the launches of a planned flush are
+    // captured into one hipGraph, which later flushes replay while their
+    // launch signature (tiles, program sizes, single-gate kernels and their
+    // arguments, columns, buffer addresses) is unchanged.  Sweep programs,
+    // which hold the gate matrices, live in the graph's own device table and
+    // are uploaded before every replay, so changed gate parameters reach a
+    // replay.  Not used while profiling or recording (events per launch), or
+    // for a flush with a single-gate kernel that uploads its matrix.
+    struct FlushGraph {
+        std::vector<uint64_t> sig;
+        hipGraphExec_t exec = nullptr;
+        void* table = nullptr;
+        size_t cap = 0;
+        RunStats delta;                  // what one replay adds to the flush's stats
+        uint64_t captures = 0, replays = 0;
+    };
+    void set_graph(bool on);
+    bool graph_enabled() const { return graph_on_ && !profiling_ && !recording_; }
+    FlushGraph& flush_graph() { return fgraph_; }
+    void drop_graph();
+    // device table of at least `bytes` for the graph (replaces the graph)
+    void* graph_table(size_t bytes);
+
     // kernel profiling (off by default): every launch of a flush is bracketed
     // by HIP events; the times are gathered after the flush synchronises.
     void set_profiling(bool on);
@@ -249,6 +272,9 @@ private:
     bool recording_ = false;
     std::vector<SweepSample> samples_;
     size_t samples_done_ = 0;             // samples whose time is known
+
+    bool graph_on_ = false;
+    FlushGraph fgraph_;
 
     bool profiling_ = false;
     std::vector<float> slot_ms_;          // per profiling slot, last collection

@@ -563,8 +563,10 @@ GpuVectorState::~GpuVectorState()
     // an own stream is destroyed below, so its work must end first; a shared
     // stream outlives the state, and the released blocks are event-guarded
     if (stream_ && !shared_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
     try {
         DeviceCache& dc = DeviceCache::get();
+        dc.release(fgraph_.table, fgraph_.cap, stream_);
         dc.release(amp_, amp_cols_ * dim() * elem_bytes(), stream_);
         dc.release(d_partial_, partial_cap_ * sizeof(double), stream_);
         dc.release(d_tree_, tree_cap_ * sizeof(double), stream_);
@@ -1404,6 +1406,36 @@ void GpuVectorState::read_amplitudes(size_t col, cdouble* out)
 
 
 // ------------------------------------------------------------ profiling
+
+void GpuVectorState::set_graph(bool on)
+{
+    graph_on_ = on;
+    if (!on) drop_graph();
+}
+
+// Every replay is launched inside flush(), which synchronises the stream
+// before it returns, so no replay is in flight here.
+void GpuVectorState::drop_graph()
+{
+    if (fgraph_.exec) (void)hipGraphExecDestroy(fgraph_.exec);
+    fgraph_.exec = nullptr;
+    fgraph_.sig.clear();
+}
+
+void* GpuVectorState::graph_table(size_t bytes)
+{
+    drop_graph();
+    if (bytes > fgraph_.cap) {
+        dev_release(fgraph_.table, fgraph_.cap);
+        fgraph_.table = nullptr;
+        fgraph_.cap = 0;
+        size_t cap = size_t(1) << 16;
+        while (cap < bytes) cap <<= 1;
+        fgraph_.table = dev_alloc(cap);
+        fgraph_.cap = cap;
+    }
+    return fgraph_.table;
+}
 
 void GpuVectorState::set_profiling(bool on)
 {

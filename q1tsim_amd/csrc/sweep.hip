@@ -901,59 +901,157 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
     }
     const size_t boff = total;
     total += nblk_floats * sizeof(float);
-    char* dprog = nullptr;
-    if (total) {
-        std::vector<char> host(total);
-        for (size_t s = 0; s < steps.size(); ++s) {
-            if (!steps[s].sweep) continue;
-            // block phases address the merged table
-            for (SwPhase& ph : progs[s].phases)
-                if (ph.kind == 1) ph.op_begin += uint32_t(boff_s[s]);
-            std::memcpy(host.data() + poff[s], progs[s].phases.data(), progs[s].phases.size() * sizeof(SwPhase));
-            std::memcpy(host.data() + ooff[s], progs[s].ops.data(), progs[s].ops.size() * sizeof(SwOp));
-            if (!progs[s].blocks.empty())
-                std::memcpy(host.data() + boff + boff_s[s] * sizeof(float), progs[s].blocks.data(),
-                            progs[s].blocks.size() * sizeof(float));
-        }
-        dprog = static_cast<char*>(st.scratch(GpuVectorState::SLOT_PROGRAM, total));
-        // ordered after the previous flush's sweeps on this (non-blocking)
-        // stream, which may still read the table
-        SW_HIP_OK(hipMemcpyAsync(dprog, host.data(), total, hipMemcpyHostToDevice, st.stream()));
-        SW_HIP_OK(hipStreamSynchronize(st.stream()));
-    }
-    const float* dblocks = reinterpret_cast<const float*>(dprog + boff);
+    std::vector<char> host(total);
     for (size_t s = 0; s < steps.size(); ++s) {
-        const PlanStep& step = steps[s];
-        if (!step.sweep) {
-            for (size_t i : step.gates) st.apply_op(queue[i].op, col_begin, col_end);
-            continue;
-        }
-        const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
-        const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
-        const int np = int(progs[s].phases.size());
-        const bool wide = has_wide_blocks(progs[s].phases);
-        int slot;
-        // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
-        if (st.precision() == PREC_C128) {
-            if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
-            else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
-        } else {
-            if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
-            else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
-            else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
-        }
-        if (st.recording() && col_end - col_begin <= 65535) {
-            float f[kSweepFeatures];
-            sweep_features(progs[s].phases, progs[s].ops, step, int(st.precision()), g, f);
-            st.add_sample(f, tiles, slot);
-        }
-        stats.launches += 1;
-        stats.sweeps += 1;
-        stats.gates += step.gates.size();
-        stats.bytes += 2.0 * S;
-        stats.phases += progs[s].phases.size();
+        if (!steps[s].sweep) continue;
+        // block phases address the merged table
+        for (SwPhase& ph : progs[s].phases)
+            if (ph.kind == 1) ph.op_begin += uint32_t(boff_s[s]);
+        std::memcpy(host.data() + poff[s], progs[s].phases.data(), progs[s].phases.size() * sizeof(SwPhase));
+        std::memcpy(host.data() + ooff[s], progs[s].ops.data(), progs[s].ops.size() * sizeof(SwOp));
+        if (!progs[s].blocks.empty())
+            std::memcpy(host.data() + boff + boff_s[s] * sizeof(float), progs[s].blocks.data(),
+                        progs[s].blocks.size() * sizeof(float));
     }
-    stats.blocks += uint64_t(nblocks);
+    // ordered after earlier work on this (non-blocking) stream, which may
+    // still read the table; the host copy must outlive the transfer
+    auto upload = [&](char* dst) {
+        if (!total) return;
+        SW_HIP_OK(hipMemcpyAsync(dst, host.data(), total, hipMemcpyHostToDevice, st.stream()));
+        SW_HIP_OK(hipStreamSynchronize(st.stream()));
+    };
+    auto launch_all = [&](char* dprog) {
+        const float* dblocks = reinterpret_cast<const float*>(dprog + boff);
+        for (size_t s = 0; s < steps.size(); ++s) {
+            const PlanStep& step = steps[s];
+            if (!step.sweep) {
+                for (size_t i : step.gates) st.apply_op(queue[i].op, col_begin, col_end);
+                continue;
+            }
+            const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
+            const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
+            const int np = int(progs[s].phases.size());
+            const bool wide = has_wide_blocks(progs[s].phases);
+            int slot;
+            // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
+            if (st.precision() == PREC_C128) {
+                if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
+                else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false);
+            } else {
+                if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
+                else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
+                else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide);
+            }
+            if (st.recording() && col_end - col_begin <= 65535) {
+                float f[kSweepFeatures];
+                sweep_features(progs[s].phases, progs[s].ops, step, int(st.precision()), g, f);
+                st.add_sample(f, tiles, slot);
+            }
+            stats.launches += 1;
+            stats.sweeps += 1;
+            stats.gates += step.gates.size();
+            stats.bytes += 2.0 * S;
+            stats.phases += progs[s].phases.size();
+        }
+        stats.blocks += uint64_t(nblocks);
+    };
+
+    bool graph = st.graph_enabled();
+    std::vector<uint64_t> sig;
+    if (graph) {
+        // the launch signature: everything a captured launch holds by value
+        auto put = [&](uint64_t x) { sig.push_back(x); };
+        put(reinterpret_cast<uintptr_t>(st.col_ptr(col_begin)));
+        put(col_begin);
+        put(col_end);
+        put(uint64_t(n));
+        put(uint64_t(st.precision()));
+        put(uint64_t(g.M));
+        put(uint64_t(g.R));
+        put(total);
+        put(boff);
+        put(steps.size());
+        for (size_t s = 0; s < steps.size() && graph; ++s) {
+            put(steps[s].sweep ? 1 : 0);
+            if (steps[s].sweep) {
+                for (int b : steps[s].tile) put(uint64_t(b));
+                put(poff[s]);
+                put(ooff[s]);
+                put(progs[s].phases.size());
+                put(has_wide_blocks(progs[s].phases) ? 1 : 0);
+                continue;
+            }
+            for (size_t i : steps[s].gates) {
+                const GateOp& op = queue[i].op;
+                // a kernel of 3+ targets uploads its matrix and synchronises
+                if (op.kind == OpKind::Dense && op.targets.size() >= 3) graph = false;
+                put(uint64_t(op.kind));
+                put(op.ctrl_mask);
+                put(op.targets.size());
+                for (int t : op.targets) put(uint64_t(t));
+                put(op.m.size());
+                for (const cdouble& c : op.m) {
+                    double re = c.real(), im = c.imag();
+                    uint64_t u;
+                    std::memcpy(&u, &re, 8);
+                    put(u);
+                    std::memcpy(&u, &im, 8);
+                    put(u);
+                }
+            }
+        }
+    }
+    if (!graph) {
+        char* dprog = total ? static_cast<char*>(st.scratch(GpuVectorState::SLOT_PROGRAM, total)) : nullptr;
+        upload(dprog);
+        launch_all(dprog);
+        return;
+    }
+
+    // graph path: replay while the signature repeats, else capture anew
+    GpuVectorState::FlushGraph& fg = st.flush_graph();
+    if (fg.exec && fg.sig == sig) {
+        upload(static_cast<char*>(fg.table));
+        SW_HIP_OK(hipGraphLaunch(fg.exec, st.stream()));
+        const RunStats& d = fg.delta;
+        stats.bytes += d.bytes;
+        stats.launches += d.launches;
+        stats.sweeps += d.sweeps;
+        stats.gates += d.gates;
+        stats.blocks += d.blocks;
+        stats.phases += d.phases;
+        fg.replays += 1;
+        return;
+    }
+    char* table = static_cast<char*>(st.graph_table(total));   // drops the old graph
+    upload(table);
+    const RunStats before = stats;
+    SW_HIP_OK(hipStreamBeginCapture(st.stream(), hipStreamCaptureModeThreadLocal));
+    try {
+        launch_all(table);
+    } catch (...) {
+        hipGraph_t part = nullptr;
+        (void)hipStreamEndCapture(st.stream(), &part);
+        if (part) (void)hipGraphDestroy(part);
+        throw;
+    }
+    hipGraph_t gr = nullptr;
+    SW_HIP_OK(hipStreamEndCapture(st.stream(), &gr));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    SW_HIP_OK(ie);
+    fg.exec = exec;
+    fg.sig = std::move(sig);
+    fg.delta = RunStats{};
+    fg.delta.bytes = stats.bytes - before.bytes;
+    fg.delta.launches = stats.launches - before.launches;
+    fg.delta.sweeps = stats.sweeps - before.sweeps;
+    fg.delta.gates = stats.gates - before.gates;
+    fg.delta.blocks = stats.blocks - before.blocks;
+    fg.delta.phases = stats.phases - before.phases;
+    fg.captures += 1;
+    SW_HIP_OK(hipGraphLaunch(exec, st.stream()));
 }
 
 // ------------------------------------------------------------- batches

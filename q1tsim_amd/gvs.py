@@ -116,6 +116,19 @@ class GpuVectorState:
         (512 MiB and more), 0 never, 1 always."""
         check(lib().q1t_gvs_set_plan_search(self.h, int(mode)))
 
+    def set_graph(self, on=True):
+        """Capture a flush's launches into a hipGraph and replay it while
+        later flushes have the same launch signature (sweep programs are
+        uploaded before each replay)."""
+        check(lib().q1t_gvs_set_graph(self.h, int(on)))
+
+    def graph_stats(self):
+        """(graphs captured, replays launched) over the state's life."""
+        c = ctypes.c_uint64(0)
+        r = ctypes.c_uint64(0)
+        check(lib().q1t_gvs_graph_stats(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return int(c.value), int(r.value)
+
     def set_geometry(self, M, R, L):
         check(lib().q1t_gvs_set_geometry(self.h, int(M), int(R), int(L)))
 

@@ -254,27 +254,84 @@ def test_search_and_swaps_keep_per_gate_result():
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_back_to_back_flushes(prec):
-    """Flushes issued one after another with nothing read between them: each
-    flush's sweep program replaces the previous one's while those sweeps may
-    still be queued, so the upload must wait for them."""
+    """Flushes one after another with no amplitudes read between them: each
+    flush's sweep program replaces the previous one's in the same device
+    table; the result is the whole gate stream's."""
     n = 20
     a = rand_state(n, 31)
     segments = [mixed_ops(n, 12, 100 + s) for s in range(10)]
     g = GpuVectorState.from_amplitudes(a, 1, prec)
     g.set_plan(4 if prec == "c64" else 0, True)
+    sweeps = 0
     for ops in segments:
         for name, bits, params in ops:
             g.apply_gate(orc.gate_matrix(name, params), bits)
         g.flush()
+        sweeps += g.stats()["sweeps"]   # stats are per flush
     ref = oracle(n, [op for ops in segments for op in ops], a)[:, 0]
     col = g.states[:, 0]
     err = float(np.abs(col - ref).max())
-    assert g.stats()["sweeps"] >= 10
+    assert sweeps >= 10
     if prec == "c128":
         assert err <= 1e-12, err
     else:
         fid = abs(np.vdot(ref, col)) ** 2 / float(np.vdot(col, col).real)
         assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+
+
+@pytest.mark.parametrize("prec,fk", [("c128", 0), ("c64", 0), ("c64", 4)])
+def test_flush_graph_replays(prec, fk):
+    """Flush graphs: one circuit shape flushed five times, twice with its own
+    angles, twice with new ones, then its own again.  The first flush
+    captures, later ones replay with their sweep programs re-uploaded; a
+    reduction and a read-back run eagerly between flushes.  Every flush
+    equals the eager path's bit for bit and the oracle."""
+    n = 18
+    a = rand_state(n, 41)
+    base = workloads.random_layered(n, 6, 3)
+    rng = np.random.Generator(np.random.PCG64(5))
+
+    def new_angles():
+        return [(g, b, [float(x) for x in rng.uniform(0.1, 6.0, len(p))]) for g, b, p in base]
+
+    rounds = [base, base, new_angles(), new_angles(), base]
+    eager = GpuVectorState.from_amplitudes(a, 1, prec)
+    graph = GpuVectorState.from_amplitudes(a, 1, prec)
+    for g in (eager, graph):
+        g.set_plan(fk, True)
+    graph.set_graph(True)
+    ref = a
+    sweeps_only = True
+    for ops in rounds:
+        for g in (eager, graph):
+            for name, bits, params in ops:
+                g.apply_gate(orc.gate_matrix(name, params), bits)
+            g.flush()
+        st = eager.stats()
+        sweeps_only = sweeps_only and st["launches"] == st["sweeps"]
+        assert graph.stats()["sweeps"] == st["sweeps"]
+        e, x = eager.states[:, 0], graph.states[:, 0]
+        assert np.array_equal(e, x)
+        assert np.array_equal(eager.marginal_w0(n - 1), graph.marginal_w0(n - 1))
+        ref = oracle(n, ops, ref)[:, 0]
+        err = float(np.abs(x - ref).max())
+        if prec == "c128":
+            assert err <= 1e-12, err
+        else:
+            fid = abs(np.vdot(ref, x)) ** 2 / float(np.vdot(x, x).real)
+            assert err <= 1e-5 and 1 - fid < 1e-6, (err, 1 - fid)
+    captures, replays = graph.graph_stats()
+    assert captures + replays == len(rounds) and captures >= 1
+    assert eager.graph_stats() == (0, 0)
+    if sweeps_only:
+        # no single-gate kernel holds a matrix by value: every later flush replays
+        assert replays == len(rounds) - 1, (captures, replays)
+
+
+def test_flush_graph_mode_is_checked():
+    g = GpuVectorState(12, 1, "c128")
+    with pytest.raises(Exception):
+        g.set_graph(2)
 
 
 def test_fused_search_c64():
