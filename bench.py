@@ -239,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the QFT, sampling, batch and learned lines")
     ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batch leg (both precisions), one JSON line; for kernel-trace profiles")
     ap.add_argument("--profile-workload", action="store_true",
                     help="one pass for rocprofv3 counter runs: k_scale over the c128 state (a read+write "
                          "of known size, for scaling FETCH_SIZE/WRITE_SIZE), then the plain circuit once")
@@ -250,6 +252,10 @@ def main():
 
     from q1tsim_amd import workloads
     from q1tsim_amd.gvs import GpuVectorState, compare
+
+    if args.batch_only:
+        print(json.dumps({"batch": batch_extra(args.batch_circuits)}))
+        return
 
     if args.profile_workload:
         n = args.n

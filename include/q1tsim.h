@@ -131,6 +131,10 @@ int q1t_batch_autotune(q1t_batch* b, size_t nr_shots, size_t sample, q1t_batch_c
                        size_t* count);
 int q1t_batch_set_config(q1t_batch* b, const q1t_batch_config* cfg);
 int q1t_batch_get_config(const q1t_batch* b, q1t_batch_config* out);
+/* Build-added, no counterpart in the reference: consecutive tiles (1..1024)
+ * a workgroup of a batched sweep takes at a time.  Schedule only; results
+ * are unchanged. */
+int q1t_batch_set_tile_chunk(q1t_batch* b, unsigned chunk);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,7 @@ SIGNATURES = {
     "q1t_batch_stats_get": (_i32, [_vp, ctypes.POINTER(BatchStats)]),
     "q1t_batch_autotune": (_i32, [_vp, _sz, _sz, ctypes.POINTER(BatchConfig), _sz, _ps]),
     "q1t_batch_set_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
+    "q1t_batch_set_tile_chunk": (_i32, [_vp, ctypes.c_uint]),
     "q1t_batch_get_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
     "q1t_gvs_new": (_i32, [_sz, _sz, _i32, ctypes.POINTER(_vp)]),
     "q1t_gvs_free": (None, [_vp]),

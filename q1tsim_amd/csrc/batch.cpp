@@ -43,6 +43,12 @@ void Batch::set_config(const TileGeom& g, int fusion_k)
     for (Circuit* c : circuits_) c->set_backend(prec_, fusion_k_, true);
 }
 
+void Batch::set_tile_chunk(unsigned c)
+{
+    if (c < 1 || c > 1024) throw Error(ERR_INVALID_ARGUMENT, "tile chunk must be 1..1024");
+    tile_chunk_ = c;
+}
+
 std::vector<BatchConfig> Batch::autotune(size_t nr_shots, size_t sample)
 {
     std::vector<BatchConfig> cand;
@@ -193,7 +199,7 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
                 states.push_back(active[k]->state());
             }
             lap(stats_.queue_s);
-            run_planned_batch(states, stream_->s, prog_, stats_.gates);
+            run_planned_batch(states, stream_->s, prog_, stats_.gates, tile_chunk_);
             stats_.rounds += 1;
             lap(stats_.flush_s);
             // Z-basis measurements of all circuits (single-qubit marginals

@@ -23,6 +23,10 @@ struct BatchStats {
     double start_s = 0.0, queue_s = 0.0, flush_s = 0.0, measure_s = 0.0, other_s = 0.0;
 };
 
+// Default consecutive tiles per workgroup step of a batched sweep (4096
+// circuits of 10-22 qubits: 16 beat 1, 4 and 64 on both precisions).
+constexpr unsigned kDefaultTileChunk = 16;
+
 // One configuration the autotuner timed.
 struct BatchConfig {
     TileGeom geom;
@@ -57,11 +61,16 @@ public:
     std::vector<BatchConfig> autotune(size_t nr_shots, size_t sample);
     void set_config(const TileGeom& g, int fusion_k);
     BatchConfig config() const { return BatchConfig{geom_, fusion_k_, 0.0}; }
+    // Consecutive tiles a workgroup of a batched sweep takes at a time (runs
+    // of one state's column share its program in the scalar cache).
+    void set_tile_chunk(unsigned c);
+    unsigned tile_chunk() const { return tile_chunk_; }
 
 private:
     Precision prec_;
     int fusion_k_;
     TileGeom geom_;
+    unsigned tile_chunk_ = kDefaultTileChunk;
     size_t mem_budget_;
     std::vector<Circuit*> circuits_;
     std::shared_ptr<SharedStream> stream_;

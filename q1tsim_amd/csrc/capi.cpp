@@ -437,6 +437,12 @@ extern "C" int q1t_batch_set_config(q1t_batch* b, const q1t_batch_config* cfg)
     return guard([&] { b->b.set_config(TileGeom{cfg->M, cfg->R, cfg->L}, cfg->fusion_k); });
 }
 
+extern "C" int q1t_batch_set_tile_chunk(q1t_batch* b, unsigned chunk)
+{
+    NEED(b);
+    return guard([&] { b->b.set_tile_chunk(chunk); });
+}
+
 extern "C" int q1t_batch_get_config(const q1t_batch* b, q1t_batch_config* out)
 {
     NEED(b);

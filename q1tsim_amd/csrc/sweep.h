@@ -117,7 +117,7 @@ struct DeviceBuffer {
 // runs in one launch over all their tiles; smaller states run whole in LDS,
 // one workgroup per column, all in one launch.
 void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
-                       RunStats& agg);
+                       RunStats& agg, unsigned tile_chunk);
 
 // Apply the pending gates to columns [col_begin, col_end) of the state with
 // the planned schedule: runs of gates whose qubits fit one tile are applied

@@ -627,18 +627,27 @@ struct SweepItem {
 template <typename V, int M, int R, int SW, bool WIDE>
 __global__ void __launch_bounds__(1 << (M - R))
 __attribute__((amdgpu_waves_per_eu(2)))
-k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_tiles)
+k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_tiles, uint32_t chunk)
 {
     __shared__ V lds[1 << M];
-    for (uint64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
-        int lo = 0, hi = nitems - 1;   // last item with first_tile <= t
+    // grid-stride over runs of `chunk` consecutive tiles: a workgroup keeps
+    // one item (and its program, in the scalar cache) for a whole run, while
+    // neighbouring workgroups still read neighbouring tiles
+    const uint64_t nchunks = (total_tiles + chunk - 1) / chunk;
+    for (uint64_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
+        const uint64_t t0 = j * chunk;
+        const uint64_t t1 = t0 + chunk < total_tiles ? t0 + chunk : total_tiles;
+        int lo = 0, hi = nitems - 1;   // last item with first_tile <= t0
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (items[mid].first_tile <= t) lo = mid;
+            if (items[mid].first_tile <= t0) lo = mid;
             else hi = mid - 1;
         }
-        const SweepItem& it = items[lo];
-        sweep_tile<V, M, R, SW, WIDE>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
+        for (uint64_t t = t0; t < t1; ++t) {
+            while (lo + 1 < nitems && items[lo + 1].first_tile <= t) ++lo;
+            const SweepItem& it = items[lo];
+            sweep_tile<V, M, R, SW, WIDE>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
+        }
     }
 }
 
@@ -1244,8 +1253,9 @@ static SmallOp make_small_op(const GateOp& g, std::vector<double2>& vals)
 }
 
 void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
-                       RunStats& agg)
+                       RunStats& agg, unsigned tile_chunk)
 {
+    if (tile_chunk == 0) throw Error(ERR_INVALID_ARGUMENT, "tile chunk must be at least 1");
     std::vector<GpuVectorState*> live;
     for (GpuVectorState* s : states)
         if (!s->pending().empty()) {
@@ -1481,18 +1491,19 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         if (!round_items[r].second) continue;
         const SweepItem* ditems = reinterpret_cast<const SweepItem*>(dbase + items_off) + round_items[r].first;
         const int nit = int(round_items[r].second);
-        const unsigned gx = unsigned(std::min<uint64_t>(round_tiles[r], 4096));
+        const uint64_t nchunks = (round_tiles[r] + tile_chunk - 1) / tile_chunk;
+        const unsigned gx = unsigned(std::min<uint64_t>(nchunks, 4096));
         if (prec == PREC_C128) {
-            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
         } else if (round_wide[r]) {
-            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, true>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, true>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
         } else {
-            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r]);
-            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r]);
+            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
         }
         SW_HIP_OK(hipGetLastError());
         agg.launches += 1;

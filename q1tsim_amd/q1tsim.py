@@ -375,6 +375,11 @@ class Batch(object):
         c = _lib.BatchConfig(M, R, L, fusion_k, 0.0)
         _lib.check(self.__sim.q1t_batch_set_config(self.__ptr, ctypes.byref(c)))
 
+    def set_tile_chunk(self, chunk):
+        """Consecutive tiles a workgroup of a batched sweep takes at a time
+        (1..1024); changes the schedule only."""
+        _lib.check(self.__sim.q1t_batch_set_tile_chunk(self.__ptr, int(chunk)))
+
     def stats(self):
         s = _lib.BatchStats()
         _lib.check(self.__sim.q1t_batch_stats_get(self.__ptr, ctypes.byref(s)))

@@ -134,3 +134,38 @@ def test_autotune_keeps_results(prec):
         solo = make(n, nc, prog, s, prec, best[3])
         solo.execute(SHOTS)
         assert np.array_equal(np.array(c.cstate()), np.array(solo.cstate())), (n, s)
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 64])
+def test_tile_chunk_keeps_results(chunk):
+    """Runs of consecutive tiles per workgroup (the batched sweep's schedule)
+    change nothing: end states equal the default schedule's bit for bit and
+    the solo path's to FP32 rounding, including chunks that straddle states
+    and a chunk longer than most states."""
+    wl = [(n, nc, prog[:-1], s) for (n, nc, prog, s) in WL]
+    runs = []
+    for ch in (None, chunk):
+        batch = Batch("c64", 0, 0)
+        circs = [make(n, nc, prog, s, "c64") for (n, nc, prog, s) in wl]
+        for c in circs:
+            batch.add(c)
+        if ch is not None:
+            batch.set_tile_chunk(ch)
+        batch.execute(SHOTS, keep_states=True)
+        runs.append(circs)
+    for (n, nc, prog, s), c0, c1 in zip(wl, runs[0], runs[1]):
+        assert c0.counts() == c1.counts(), (n, s)
+        for col in range(c1.nr_columns()):
+            assert np.array_equal(c0.amplitudes(col), c1.amplitudes(col)), (n, s, col)
+        solo = make(n, nc, prog, s, "c64")
+        solo.execute(SHOTS)
+        assert c1.counts() == solo.counts(), (n, s)
+        for col in range(c1.nr_columns()):
+            assert np.max(np.abs(c1.amplitudes(col) - solo.amplitudes(col))) <= 1e-5, (n, s, col)
+
+
+def test_tile_chunk_is_checked():
+    batch = Batch("c64", 0, 0)
+    for bad in (0, 2000):
+        with pytest.raises(Exception):
+            batch.set_tile_chunk(bad)
