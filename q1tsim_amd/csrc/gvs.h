@@ -72,20 +72,22 @@ struct PendingGate {
 // a batch come and go with the same few sizes).  A released block carries an
 // event recorded on its stream: the same stream reuses it at once (stream
 // order), another stream only once the event has completed.  Cached bytes
-// are capped; on allocation failure the cache is emptied and the allocation
-// retried.
+// are capped: a release over the cap frees the oldest cached blocks first.
+// On allocation failure the cache is emptied and the allocation retried.
 class DeviceCache {
 public:
     static DeviceCache& get();
     void* alloc(size_t bytes, hipStream_t s);
     void release(void* p, size_t bytes, hipStream_t s);
     void trim();
+    // `s` has been synchronised and is about to be destroyed
+    void forget_stream(hipStream_t s);
 
 private:
     struct Entry {
         void* p;
         size_t bytes;
-        hipStream_t s;
+        hipStream_t s;    // null with ev: idle, reusable on any stream
         hipEvent_t ev;
     };
     std::vector<Entry> free_;

@@ -447,12 +447,27 @@ void DeviceCache::trim()
 {
     std::lock_guard<std::mutex> lk(mu_);
     for (Entry& e : free_) {
-        HIP_OK(hipEventSynchronize(e.ev));
-        HIP_OK(hipEventDestroy(e.ev));
+        if (e.ev) {
+            HIP_OK(hipEventSynchronize(e.ev));
+            HIP_OK(hipEventDestroy(e.ev));
+        }
         HIP_OK(hipFree(e.p));
     }
     free_.clear();
     cached_ = 0;
+}
+
+// Called with `s` synchronised and before it is destroyed: its blocks are
+// idle, and their events must not outlive the stream they were recorded on.
+void DeviceCache::forget_stream(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    for (Entry& e : free_) {
+        if (e.s != s) continue;
+        if (e.ev) (void)hipEventDestroy(e.ev);
+        e.ev = nullptr;
+        e.s = nullptr;
+    }
 }
 
 void* DeviceCache::alloc(size_t bytes, hipStream_t s)
@@ -462,15 +477,14 @@ void* DeviceCache::alloc(size_t bytes, hipStream_t s)
         for (size_t i = 0; i < free_.size(); ++i) {
             Entry& e = free_[i];
             if (e.bytes != bytes) continue;
-            if (e.s != s && hipEventQuery(e.ev) != hipSuccess) {
+            if (e.ev && e.s != s && hipEventQuery(e.ev) != hipSuccess) {
                 (void)hipGetLastError();   // a pending event is not an error: keep it out of the next check
                 continue;
             }
             void* p = e.p;
-            HIP_OK(hipEventDestroy(e.ev));
+            if (e.ev) HIP_OK(hipEventDestroy(e.ev));
             cached_ -= e.bytes;
-            free_[i] = free_.back();
-            free_.pop_back();
+            free_.erase(free_.begin() + std::ptrdiff_t(i));   // keeps release order (oldest first)
             return p;
         }
     }
@@ -498,10 +512,22 @@ void DeviceCache::release(void* p, size_t bytes, hipStream_t s)
         HIP_OK(hipMemGetInfo(&f, &t));
         cap_ = t / 8;
     }
-    if (cached_ + bytes > cap_) {
+    if (bytes > cap_) {
         HIP_OK(hipStreamSynchronize(s));
         HIP_OK(hipFree(p));
         return;
+    }
+    // over the cap: free the oldest cached blocks, so the cache follows the
+    // sizes in use now (a batch after a large state, c128 after c64)
+    while (cached_ + bytes > cap_ && !free_.empty()) {
+        Entry& e = free_.front();
+        if (e.ev) {
+            HIP_OK(hipEventSynchronize(e.ev));
+            HIP_OK(hipEventDestroy(e.ev));
+        }
+        HIP_OK(hipFree(e.p));
+        cached_ -= e.bytes;
+        free_.erase(free_.begin());
     }
     hipEvent_t ev;
     HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -516,6 +542,7 @@ SharedStream::~SharedStream()
 {
     if (s) {
         (void)hipStreamSynchronize(s);
+        DeviceCache::get().forget_stream(s);
         (void)hipStreamDestroy(s);
     }
 }
@@ -578,7 +605,10 @@ GpuVectorState::~GpuVectorState()
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
-    if (stream_ && !shared_) (void)hipStreamDestroy(stream_);
+    if (stream_ && !shared_) {
+        DeviceCache::get().forget_stream(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
 }
 
 void GpuVectorState::check_qbit(size_t q) const
