@@ -73,6 +73,12 @@ result_t circuit_c_qasm(const circuit_t* ptr);                        /* ffi.rs:
 /* ---- build-added controls (no ffi.rs counterpart) ---- */
 /* precision Q1T_PREC_C128 (plain, FP64) or Q1T_PREC_C64; see q1t_gvs_set_plan */
 int q1t_circuit_set_backend(circuit_t* ptr, int precision, int fusion_k, int tiling);
+/* Build-added, no counterpart in the reference: flush graphs of the
+ * circuit's state (q1t_gvs_set_graph).  `reexecute` keeps the state, so its
+ * gate flushes replay the previous execute's graphs, with reference
+ * parameters re-read; graph_stats counts over the current state. */
+int q1t_circuit_set_graph(circuit_t* ptr, int on);
+int q1t_circuit_graph_stats(circuit_t* ptr, uint64_t* captures, uint64_t* replays);
 /* seed of the built-in xoshiro256** stream used by execute/reexecute */
 int q1t_circuit_seed(circuit_t* ptr, uint64_t seed);
 /* initial state for execute: 2^n interleaved amplitudes, or a seeded N(0,1) state */

@@ -82,6 +82,8 @@ SIGNATURES = {
     "q1t_last_error": (_cs, []),
     "q1t_circuit_set_backend": (_i32, [_vp, _i32, _i32, _i32]),
     "q1t_circuit_seed": (_i32, [_vp, _u64]),
+    "q1t_circuit_set_graph": (_i32, [_vp, _i32]),
+    "q1t_circuit_graph_stats": (_i32, [_vp, _pu64, _pu64]),
     "q1t_circuit_set_initial_amplitudes": (_i32, [_vp, _pd, _sz]),
     "q1t_circuit_set_initial_random": (_i32, [_vp, _u64]),
     "q1t_circuit_barrier": (_i32, [_vp, _ps, _sz]),

@@ -259,6 +259,26 @@ extern "C" int q1t_circuit_set_backend(circuit_t* ptr, int precision, int fusion
     });
 }
 
+extern "C" int q1t_circuit_set_graph(circuit_t* ptr, int on)
+{
+    NEED(ptr);
+    return guard([&] {
+        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "graph mode must be 0 or 1");
+        ptr->c.set_graph(on != 0);
+    });
+}
+
+extern "C" int q1t_circuit_graph_stats(circuit_t* ptr, uint64_t* captures, uint64_t* replays)
+{
+    NEED(ptr);
+    NEED(captures);
+    NEED(replays);
+    GpuVectorState* st = ptr->c.state();
+    *captures = st ? st->flush_graph().captures : 0;
+    *replays = st ? st->flush_graph().replays : 0;
+    return 0;
+}
+
 extern "C" int q1t_circuit_seed(circuit_t* ptr, uint64_t seed)
 {
     NEED(ptr);

@@ -85,6 +85,13 @@ public:
 
     // build-added
     void set_backend(Precision prec, int fusion_k, bool tiling);
+    // flush graphs of the state (GpuVectorState::set_graph); a reexecute
+    // keeps the state, so its flushes can replay the previous execute's
+    void set_graph(bool on)
+    {
+        graph_ = on;
+        if (state_) state_->set_graph(on);
+    }
     void seed(uint64_t s) { rng_.reseed(s); }
     Xoshiro256 rng_state() const { return rng_; }
     void set_rng_state(const Xoshiro256& r) { rng_ = r; }
@@ -104,6 +111,7 @@ private:
     Precision prec_ = PREC_C128;
     int fusion_k_ = 0;
     bool tiling_ = true;
+    bool graph_ = false;
     int init_mode_ = 0;            // 0: |0..0>, 1: amplitudes, 2: random normal
     std::vector<cdouble> init_amps_;
     uint64_t init_seed_ = 0;

@@ -262,6 +262,17 @@ class Circuit(object):
         prec = {"c128": 0, "c64": 1}[precision]
         _lib.check(self.__sim.q1t_circuit_set_backend(self.__ptr, prec, int(fusion_k), 1 if tiling else 0))
 
+    def set_graph(self, on=True):
+        """Flush graphs of the state: gate flushes of a reexecute replay the
+        previous execute's, with reference parameters re-read."""
+        _lib.check(self.__sim.q1t_circuit_set_graph(self.__ptr, 1 if on else 0))
+
+    def graph_stats(self):
+        """(graphs captured, replays launched) over the current state."""
+        c, r = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _lib.check(self.__sim.q1t_circuit_graph_stats(self.__ptr, ctypes.byref(c), ctypes.byref(r)))
+        return int(c.value), int(r.value)
+
     def seed(self, seed):
         _lib.check(self.__sim.q1t_circuit_seed(self.__ptr, int(seed)))
 

@@ -129,6 +129,43 @@ def test_reexecute_and_reference_parameters():
     assert abs(p2 - math.sin((math.pi + 1.23) / 2) ** 2) < 0.05
 
 
+def test_reexecute_replays_flush_graphs():
+    """Flush graphs on a circuit (hipGraph): reexecute keeps the state, so its
+    gate flush replays the previous execute's graph, with the angles of the
+    reference parameters re-read each time; every pass equals the same
+    circuit run without graphs, bit for bit."""
+    n = 16
+    base = workloads.random_layered(n, 4, 2)
+    runs = []
+    for graph in (False, True):
+        refs = []
+        c = Circuit(n, 1)
+        for name, bits, params in base:
+            if params:
+                r = RefParam(params[0])
+                refs.append(r)
+                c.add_gate(name, bits, [r])
+            else:
+                c.add_gate(name, bits)
+        c.peek(0, 0)
+        c.seed(4)
+        c.set_graph(graph)
+        c.execute(8)
+        amps = [c.amplitudes(0)]
+        rng = np.random.Generator(np.random.PCG64(9))
+        for _ in range(3):
+            for r in refs:
+                r.assign(float(rng.uniform(0.1, 6.0)))
+            c.reexecute()
+            amps.append(c.amplitudes(0))
+        runs.append((amps, c.graph_stats(), c))
+    (a0, g0, _), (a1, g1, _) = runs
+    for x, y in zip(a0, a1):
+        assert np.array_equal(x, y)
+    assert g0 == (0, 0)
+    assert g1[0] >= 1 and g1[0] + g1[1] == 4, g1
+
+
 def test_cstate_script():
     """python/test_cstate.py, with the binomial band of stats.rs."""
     with Circuit(2, 2) as c:
