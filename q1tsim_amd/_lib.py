@@ -147,6 +147,8 @@ SIGNATURES = {
     "q1t_gvs_set_profiling": (_i32, [_vp, _i32]),
     "q1t_gvs_kernel_profile": (_i32, [_vp, _vp, _sz, _ps]),
     "q1t_copy_bandwidth": (_i32, [_sz, _i32, _pd]),
+    "q1t_device_cache_limit": (_i32, [_sz]),
+    "q1t_device_cache_bytes": (_i32, [_ps]),
     "q1t_gvs_compare": (_i32, [_vp, _sz, _vp, _sz, _pd]),
     "q1t_gate_matrix": (_i32, [_cs, _pd, _sz, _pd, _sz, _ps]),
     "q1t_xoshiro_new": (_vp, [_u64]),

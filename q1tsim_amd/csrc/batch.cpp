@@ -162,7 +162,8 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
     if (!budget) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) throw Error(ERR_HIP, "hipMemGetInfo failed");
-        budget = free_b / 4 * 3;
+        // blocks held by the device block cache are free for the batch
+        budget = (free_b + DeviceCache::get().cached_bytes()) / 4 * 3;
     }
     size_t next = 0;
     while (next < list.size()) {

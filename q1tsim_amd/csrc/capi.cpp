@@ -763,6 +763,17 @@ extern "C" int q1t_gvs_set_profiling(q1t_gvs* s, int on)
     return guard([&] { s->s.set_profiling(on != 0); });
 }
 
+extern "C" int q1t_device_cache_limit(size_t bytes)
+{
+    return guard([&] { DeviceCache::get().set_cap(bytes); });
+}
+
+extern "C" int q1t_device_cache_bytes(size_t* out)
+{
+    NEED(out);
+    return guard([&] { *out = DeviceCache::get().cached_bytes(); });
+}
+
 extern "C" int q1t_copy_bandwidth(size_t bytes, int reps, double* best_ms)
 {
     NEED(best_ms);

@@ -72,8 +72,10 @@ struct PendingGate {
 // a batch come and go with the same few sizes).  A released block carries an
 // event recorded on its stream: the same stream reuses it at once (stream
 // order), another stream only once the event has completed.  Cached bytes
-// are capped: a release over the cap frees the oldest cached blocks first.
-// On allocation failure the cache is emptied and the allocation retried.
+// are capped (default: half of device memory; a batch's states come and go
+// by the hundreds of GB): a release over the cap frees the oldest cached
+// blocks first.  On allocation failure the cache is emptied and the
+// allocation retried.
 class DeviceCache {
 public:
     static DeviceCache& get();
@@ -82,8 +84,13 @@ public:
     void trim();
     // `s` has been synchronised and is about to be destroyed
     void forget_stream(hipStream_t s);
+    // cap on cached bytes (0: the default), evicting down to it now
+    void set_cap(size_t bytes);
+    size_t cached_bytes();
 
 private:
+    static size_t default_cap();
+    void evict_to(size_t limit);   // caller holds mu_
     struct Entry {
         void* p;
         size_t bytes;

@@ -457,6 +457,40 @@ void DeviceCache::trim()
     cached_ = 0;
 }
 
+size_t DeviceCache::default_cap()
+{
+    size_t f = 0, t = 0;
+    HIP_OK(hipMemGetInfo(&f, &t));
+    return t / 2;
+}
+
+void DeviceCache::evict_to(size_t limit)
+{
+    while (cached_ > limit && !free_.empty()) {
+        Entry& e = free_.front();
+        if (e.ev) {
+            HIP_OK(hipEventSynchronize(e.ev));
+            HIP_OK(hipEventDestroy(e.ev));
+        }
+        HIP_OK(hipFree(e.p));
+        cached_ -= e.bytes;
+        free_.erase(free_.begin());
+    }
+}
+
+void DeviceCache::set_cap(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    cap_ = bytes ? bytes : default_cap();
+    evict_to(cap_);
+}
+
+size_t DeviceCache::cached_bytes()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
+}
+
 // Called with `s` synchronised and before it is destroyed: its blocks are
 // idle, and their events must not outlive the stream they were recorded on.
 void DeviceCache::forget_stream(hipStream_t s)
@@ -507,11 +541,7 @@ void DeviceCache::release(void* p, size_t bytes, hipStream_t s)
 {
     if (!p) return;
     std::lock_guard<std::mutex> lk(mu_);
-    if (!cap_) {
-        size_t f = 0, t = 0;
-        HIP_OK(hipMemGetInfo(&f, &t));
-        cap_ = t / 8;
-    }
+    if (!cap_) cap_ = default_cap();
     if (bytes > cap_) {
         HIP_OK(hipStreamSynchronize(s));
         HIP_OK(hipFree(p));
@@ -519,16 +549,7 @@ void DeviceCache::release(void* p, size_t bytes, hipStream_t s)
     }
     // over the cap: free the oldest cached blocks, so the cache follows the
     // sizes in use now (a batch after a large state, c128 after c64)
-    while (cached_ + bytes > cap_ && !free_.empty()) {
-        Entry& e = free_.front();
-        if (e.ev) {
-            HIP_OK(hipEventSynchronize(e.ev));
-            HIP_OK(hipEventDestroy(e.ev));
-        }
-        HIP_OK(hipFree(e.p));
-        cached_ -= e.bytes;
-        free_.erase(free_.begin());
-    }
+    evict_to(cap_ - bytes);
     hipEvent_t ev;
     HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIP_OK(hipEventRecord(ev, s));

@@ -50,6 +50,18 @@ def copy_bandwidth(nbytes=8 << 30, reps=5):
     return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+def set_device_cache_limit(nbytes=0):
+    """Cap on released device blocks kept for reuse (0: half of device
+    memory); lowering it frees the oldest cached blocks now."""
+    check(lib().q1t_device_cache_limit(int(nbytes)))
+
+
+def device_cache_bytes():
+    out = ctypes.c_size_t(0)
+    check(lib().q1t_device_cache_bytes(ctypes.byref(out)))
+    return int(out.value)
+
+
 def compare(a, b, col_a=0, col_b=0):
     """(infidelity, max-abs difference) between one column of each state (FP64 on device)."""
     out = np.zeros(5, dtype=np.float64)

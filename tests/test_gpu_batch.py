@@ -174,19 +174,26 @@ def test_tile_chunk_is_checked():
 def test_block_cache_after_stream_teardown():
     """Blocks cached from streams that no longer exist (a finished batch's
     shared stream, states' own streams) are reused or freed safely when later
-    releases push the cache over its cap (1/8 of device memory)."""
+    releases push the cache over its limit (set to 36 GiB here)."""
     import gc
-    from q1tsim_amd.gvs import GpuVectorState
-    batch, circs = run_batch(WL[:6], "c64")
-    del batch, circs
-    gc.collect()
-    a, b = GpuVectorState(30, 1, "c128"), GpuVectorState(30, 1, "c128")   # 16 GiB each, both live
-    a.close()
-    b.close()
-    c = GpuVectorState(29, 1, "c128")   # 8 GiB, a new size: its release goes over the cap
-    c.close()
-    batch, circs = run_batch(WL, "c128")
-    for (n, nc, prog, s), c in zip(WL, circs):
-        solo = make(n, nc, prog, s, "c128")
-        solo.execute(SHOTS)
-        assert np.array_equal(np.array(c.cstate()), np.array(solo.cstate())), (n, s)
+    from q1tsim_amd.gvs import GpuVectorState, device_cache_bytes, set_device_cache_limit
+    set_device_cache_limit(36 << 30)
+    try:
+        batch, circs = run_batch(WL[:6], "c64")
+        del batch, circs
+        gc.collect()
+        a, b = GpuVectorState(30, 1, "c128"), GpuVectorState(30, 1, "c128")   # 16 GiB each, both live
+        a.close()
+        b.close()
+        c = GpuVectorState(29, 1, "c128")   # 8 GiB, a new size: its release goes over the limit
+        c.close()
+        assert device_cache_bytes() <= 36 << 30
+        batch, circs = run_batch(WL, "c128")
+        for (n, nc, prog, s), c in zip(WL, circs):
+            solo = make(n, nc, prog, s, "c128")
+            solo.execute(SHOTS)
+            assert np.array_equal(np.array(c.cstate()), np.array(solo.cstate())), (n, s)
+        set_device_cache_limit(1 << 30)   # lowering the limit frees down to it at once
+        assert device_cache_bytes() <= 1 << 30
+    finally:
+        set_device_cache_limit(0)
