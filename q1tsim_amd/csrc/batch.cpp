@@ -125,10 +125,19 @@ size_t Batch::resident_bytes(const Circuit& c, size_t nr_shots) const
             break;
         }
         case OpType::MeasureAll: {
-            bool gates_follow = false;
-            for (size_t j = i + 1; j < ops.size(); ++j)
-                gates_follow = gates_follow || ops[j].type == OpType::Gate || ops[j].type == OpType::CondGate;
-            if (gates_follow) cols = nr_shots;
+            // The basis columns become dense (up to nr_shots of them) when
+            // gates touch them: this op's own X/Y restoring gates, a later
+            // gate or conditional gate, a reset (its conditional X), or a
+            // later X/Y-basis measurement (its basis change), up to the next
+            // reset_all.
+            bool dense = op.basis != 'Z';
+            for (size_t j = i + 1; j < ops.size() && !dense && ops[j].type != OpType::ResetAll; ++j) {
+                const CircuitOp& o = ops[j];
+                dense = o.type == OpType::Gate || o.type == OpType::CondGate || o.type == OpType::Reset ||
+                        ((o.type == OpType::Measure || o.type == OpType::Peek || o.type == OpType::MeasureAll ||
+                          o.type == OpType::PeekAll) && o.basis != 'Z');
+            }
+            if (dense) cols = nr_shots;
             for (size_t b : op.cbits) det[b] = 1;
             break;
         }

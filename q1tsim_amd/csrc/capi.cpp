@@ -274,8 +274,8 @@ extern "C" int q1t_circuit_graph_stats(circuit_t* ptr, uint64_t* captures, uint6
     NEED(captures);
     NEED(replays);
     GpuVectorState* st = ptr->c.state();
-    *captures = st ? st->flush_graph().captures : 0;
-    *replays = st ? st->flush_graph().replays : 0;
+    *captures = st ? st->graph_captures() : 0;
+    *replays = st ? st->graph_replays() : 0;
     return 0;
 }
 
@@ -716,8 +716,8 @@ extern "C" int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* rep
     NEED(s);
     NEED(captures);
     NEED(replays);
-    *captures = s->s.flush_graph().captures;
-    *replays = s->s.flush_graph().replays;
+    *captures = s->s.graph_captures();
+    *replays = s->s.graph_replays();
     return 0;
 }
 

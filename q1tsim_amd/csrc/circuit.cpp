@@ -336,7 +336,8 @@ std::vector<std::pair<std::string, size_t>> Circuit::histogram_string() const
         size_t width = nr_cbits_;
         size_t nb = 0;
         for (uint64_t t = k; t; t >>= 1) ++nb;
-        size_t len = std::max(width, nb);
+        // {:0width$b} prints at least one digit: key 0 with width 0 is "0"
+        size_t len = std::max(width, std::max<size_t>(nb, 1));
         s.assign(len, '0');
         for (size_t b = 0; b < len; ++b)
             if ((k >> b) & 1) s[len - 1 - b] = '1';

@@ -224,21 +224,31 @@ public:
     // which hold the gate matrices, live in the graph's own device table and
     // are uploaded before every replay, so changed gate parameters reach a
     // replay.  Not used while profiling or recording (events per launch), or
-    // for a flush with a single-gate kernel that uploads its matrix.
+    // for a flush with a single-gate kernel that uploads its matrix.  A state
+    // keeps up to kMaxFlushGraphs graphs, one per distinct signature (the
+    // flushes between the measurements of a circuit differ), and evicts the
+    // least recently used one.
     struct FlushGraph {
         std::vector<uint64_t> sig;
         hipGraphExec_t exec = nullptr;
         void* table = nullptr;
         size_t cap = 0;
         RunStats delta;                  // what one replay adds to the flush's stats
-        uint64_t captures = 0, replays = 0;
+        uint64_t last_use = 0;
     };
+    static constexpr size_t kMaxFlushGraphs = 8;
     void set_graph(bool on);
     bool graph_enabled() const { return graph_on_ && !profiling_ && !recording_; }
-    FlushGraph& flush_graph() { return fgraph_; }
-    void drop_graph();
-    // device table of at least `bytes` for the graph (replaces the graph)
-    void* graph_table(size_t bytes);
+    // the graph captured for this signature, or null
+    FlushGraph* find_graph(const std::vector<uint64_t>& sig);
+    // an empty slot whose device table holds at least `bytes` (evicts the
+    // least recently used graph when all slots are taken)
+    FlushGraph& new_graph_slot(size_t bytes);
+    void note_graph(bool captured) { (captured ? graph_captures_ : graph_replays_) += 1; }
+    uint64_t graph_captures() const { return graph_captures_; }
+    uint64_t graph_replays() const { return graph_replays_; }
+    size_t graph_count() const;
+    void drop_graphs();
 
     // kernel profiling (off by default): every launch of a flush is bracketed
     // by HIP events; the times are gathered after the flush synchronises.
@@ -283,7 +293,8 @@ private:
     size_t samples_done_ = 0;             // samples whose time is known
 
     bool graph_on_ = false;
-    FlushGraph fgraph_;
+    std::vector<FlushGraph> fgraphs_;
+    uint64_t graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;
 
     bool profiling_ = false;
     std::vector<float> slot_ms_;          // per profiling slot, last collection

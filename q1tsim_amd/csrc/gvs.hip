@@ -611,10 +611,13 @@ GpuVectorState::~GpuVectorState()
     // an own stream is destroyed below, so its work must end first; a shared
     // stream outlives the state, and the released blocks are event-guarded
     if (stream_ && !shared_) (void)hipStreamSynchronize(stream_);
-    drop_graph();
     try {
         DeviceCache& dc = DeviceCache::get();
-        dc.release(fgraph_.table, fgraph_.cap, stream_);
+        for (FlushGraph& fg : fgraphs_) {
+            if (fg.exec) (void)hipGraphExecDestroy(fg.exec);
+            dc.release(fg.table, fg.cap, stream_);
+        }
+        fgraphs_.clear();
         dc.release(amp_, amp_cols_ * dim() * elem_bytes(), stream_);
         dc.release(d_partial_, partial_cap_ * sizeof(double), stream_);
         dc.release(d_tree_, tree_cap_ * sizeof(double), stream_);
@@ -791,8 +794,9 @@ void GpuVectorState::materialize()
     const size_t bytes_needed = nc * dim() * elem_bytes();
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    // blocks idle in the device cache are freed by its allocation retry, so they count as free
     size_t have = amp_ ? amp_cols_ * dim() * elem_bytes() : 0;
-    if (bytes_needed > free_b + have)
+    if (bytes_needed > free_b + have + DeviceCache::get().cached_bytes())
         throw Error(ERR_OUT_OF_MEMORY, "Not enough device memory to expand " + std::to_string(nc) +
                                            " measured basis columns into dense columns");
     alloc_cols(nc);
@@ -1461,31 +1465,63 @@ void GpuVectorState::read_amplitudes(size_t col, cdouble* out)
 void GpuVectorState::set_graph(bool on)
 {
     graph_on_ = on;
-    if (!on) drop_graph();
+    if (!on) drop_graphs();
 }
 
 // Every replay is launched inside flush(), which synchronises the stream
-// before it returns, so no replay is in flight here.
-void GpuVectorState::drop_graph()
+// before it returns, so no replay is in flight here.  The tables go back to
+// the device block cache, guarded by an event on the stream.
+void GpuVectorState::drop_graphs()
 {
-    if (fgraph_.exec) (void)hipGraphExecDestroy(fgraph_.exec);
-    fgraph_.exec = nullptr;
-    fgraph_.sig.clear();
+    for (FlushGraph& fg : fgraphs_) {
+        if (fg.exec) (void)hipGraphExecDestroy(fg.exec);
+        dev_release(fg.table, fg.cap);
+    }
+    fgraphs_.clear();
 }
 
-void* GpuVectorState::graph_table(size_t bytes)
+size_t GpuVectorState::graph_count() const
 {
-    drop_graph();
-    if (bytes > fgraph_.cap) {
-        dev_release(fgraph_.table, fgraph_.cap);
-        fgraph_.table = nullptr;
-        fgraph_.cap = 0;
+    size_t n = 0;
+    for (const FlushGraph& fg : fgraphs_) n += fg.exec != nullptr;
+    return n;
+}
+
+GpuVectorState::FlushGraph* GpuVectorState::find_graph(const std::vector<uint64_t>& sig)
+{
+    for (FlushGraph& fg : fgraphs_)
+        if (fg.exec && fg.sig == sig) {
+            fg.last_use = ++graph_clock_;
+            return &fg;
+        }
+    return nullptr;
+}
+
+GpuVectorState::FlushGraph& GpuVectorState::new_graph_slot(size_t bytes)
+{
+    FlushGraph* slot = nullptr;
+    if (fgraphs_.size() < kMaxFlushGraphs) {
+        fgraphs_.emplace_back();
+        slot = &fgraphs_.back();
+    } else {
+        slot = &fgraphs_[0];
+        for (FlushGraph& fg : fgraphs_)
+            if (fg.last_use < slot->last_use) slot = &fg;
+    }
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    slot->exec = nullptr;
+    slot->sig.clear();
+    if (bytes > slot->cap) {
+        dev_release(slot->table, slot->cap);
+        slot->table = nullptr;
+        slot->cap = 0;
         size_t cap = size_t(1) << 16;
         while (cap < bytes) cap <<= 1;
-        fgraph_.table = dev_alloc(cap);
-        fgraph_.cap = cap;
+        slot->table = dev_alloc(cap);
+        slot->cap = cap;
     }
-    return fgraph_.table;
+    slot->last_use = ++graph_clock_;
+    return *slot;
 }
 
 void GpuVectorState::set_profiling(bool on)

@@ -1018,21 +1018,21 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
     }
 
     // graph path: replay while the signature repeats, else capture anew
-    GpuVectorState::FlushGraph& fg = st.flush_graph();
-    if (fg.exec && fg.sig == sig) {
-        upload(static_cast<char*>(fg.table));
-        SW_HIP_OK(hipGraphLaunch(fg.exec, st.stream()));
-        const RunStats& d = fg.delta;
+    if (GpuVectorState::FlushGraph* hit = st.find_graph(sig)) {
+        upload(static_cast<char*>(hit->table));
+        SW_HIP_OK(hipGraphLaunch(hit->exec, st.stream()));
+        const RunStats& d = hit->delta;
         stats.bytes += d.bytes;
         stats.launches += d.launches;
         stats.sweeps += d.sweeps;
         stats.gates += d.gates;
         stats.blocks += d.blocks;
         stats.phases += d.phases;
-        fg.replays += 1;
+        st.note_graph(false);
         return;
     }
-    char* table = static_cast<char*>(st.graph_table(total));   // drops the old graph
+    GpuVectorState::FlushGraph& fg = st.new_graph_slot(total);
+    char* table = static_cast<char*>(fg.table);
     upload(table);
     const RunStats before = stats;
     SW_HIP_OK(hipStreamBeginCapture(st.stream(), hipStreamCaptureModeThreadLocal));
@@ -1059,7 +1059,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
     fg.delta.gates = stats.gates - before.gates;
     fg.delta.blocks = stats.blocks - before.blocks;
     fg.delta.phases = stats.phases - before.phases;
-    fg.captures += 1;
+    st.note_graph(true);
     SW_HIP_OK(hipGraphLaunch(exec, st.stream()));
 }
 

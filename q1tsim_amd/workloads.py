@@ -127,8 +127,9 @@ def batch_workload(count, seed, qubits=(10, 22), depths=(10, 40), meas=(1, 3)):
 
 
 def build_program(circ, program):
-    """Add a program (as `branching` returns it) to a circuit object with
-    q1tsim's add_gate / add_conditional_gate / measure / measure_all API."""
+    """Add a program (as `branching` returns it, plus "peek", "peek_all",
+    "reset" and "reset_all" entries) to a circuit object with q1tsim's
+    add_gate / add_conditional_gate / measure / peek / reset API."""
     for op in program:
         kind = op[0]
         if kind == "gate":
@@ -139,6 +140,14 @@ def build_program(circ, program):
             circ.measure_basis(op[1], op[2], op[3])
         elif kind == "measure_all":
             circ.measure_all_basis(op[1], op[2])
+        elif kind == "peek":
+            circ.peek_basis(op[1], op[2], op[3])
+        elif kind == "peek_all":
+            circ.peek_all_basis(op[1], op[2])
+        elif kind == "reset":
+            circ.reset(op[1])
+        elif kind == "reset_all":
+            circ.reset_all()
         else:
             raise ValueError("unknown program op %r" % (kind,))
     return circ

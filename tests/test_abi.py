@@ -109,3 +109,42 @@ def test_no_gpu_fails_loudly():
         c.h(0)
         with pytest.raises(Q1tError, match="HIP"):
             c.execute(10)
+
+
+ALL_NAMES = [("i", 0), ("h", 0), ("x", 0), ("y", 0), ("z", 0), ("s", 0), ("sdg", 0), ("t", 0), ("tdg", 0),
+             ("rx", 1), ("ry", 1), ("rz", 1), ("u1", 1), ("u2", 2), ("u3", 3), ("v", 0), ("vdg", 0),
+             ("swap", 0), ("cx", 0), ("cy", 0), ("cz", 0), ("ch", 0), ("crx", 1), ("cry", 1), ("crz", 1),
+             ("cs", 0), ("csdg", 0), ("ct", 0), ("ctdg", 0), ("cu1", 1), ("cu2", 2), ("cu3", 3), ("cv", 0),
+             ("cvdg", 0), ("ccx", 0), ("ccz", 0), ("ccrx", 1), ("ccry", 1), ("ccrz", 1)]
+
+
+def test_product_gate_table_matches_closed_forms():
+    """The library's own gate table (the one Circuit.add_gate uses) against
+    the independent closed forms of tests/golden/dense_sim.py, every name,
+    several random parameter sets (distinct values, so a swapped parameter
+    order fails)."""
+    import numpy as np
+
+    from q1tsim_amd.gvs import gate_matrix
+    from tests.golden import dense_sim
+
+    rng = np.random.Generator(np.random.PCG64(17))
+    for name, npar in ALL_NAMES:
+        for _ in range(4 if npar else 1):
+            p = [float(x) for x in rng.uniform(-2 * np.pi, 2 * np.pi, size=npar)]
+            got = gate_matrix(name, p)
+            exp = dense_sim.gate_matrix(name, p)
+            assert got.shape == exp.shape, name
+            assert np.abs(got - exp).max() < 1e-15, (name, p)
+            # upper-case names are accepted too (ffi.rs matches case-insensitively)
+            assert np.array_equal(gate_matrix(name.upper(), p), got)
+
+
+def test_product_gate_table_is_complete():
+    """Every name of ffi.rs:335-366 plus the composite names the QFT needs."""
+    from q1tsim_amd.gvs import gate_matrix
+    for name, npar in ALL_NAMES:
+        gate_matrix(name, [0.1 * (k + 1) for k in range(npar)])
+    with pytest.raises(Q1tError, match='Unknown gate "cxx"'):
+        gate_matrix("cxx", [])
+

@@ -34,6 +34,14 @@ def make_ref(n, nc, prog):
             r.measure(op[1], op[2], op[3])
         elif op[0] == "measure_all":
             r.measure_all(op[1], op[2])
+        elif op[0] == "peek":
+            r.peek(op[1], op[2], op[3])
+        elif op[0] == "peek_all":
+            r.peek_all(op[1], op[2])
+        elif op[0] == "reset":
+            r.reset(op[1])
+        elif op[0] == "reset_all":
+            r.reset_all()
     return r
 
 
@@ -101,6 +109,38 @@ def test_batch_waves_under_small_budget():
     assert small.stats()["waves"] > 1 and big.stats()["waves"] == 1
     for a, b in zip(circs_big, circs_small):
         assert np.array_equal(np.array(a.cstate()), np.array(b.cstate()))
+
+
+def test_batch_budget_counts_dense_basis_columns():
+    """After a collapsing measure_all, X/Y restoring gates, a reset, or a
+    later X/Y measurement turn the run's basis columns into up to nr_shots
+    dense columns.  The wave planner must count them: 12-qubit circuits with
+    64 runs take 64 x 64 KiB = 4 MiB each, so a 10 MiB budget holds at most
+    two per wave.  Classical states equal the solo runs and the oracle's."""
+    n = 12
+    wl = []
+    tails = [
+        [("measure_all", list(range(n)), "X")],
+        [("measure_all", list(range(n)), "Y"), ("peek_all", list(range(n, 2 * n)), "X")],
+        [("measure_all", list(range(n)), "Z"), ("reset", 1), ("measure", 1, n, "Z")],
+        [("measure_all", list(range(n)), "Z"), ("measure", 2, n, "Y")],
+        [("measure_all", list(range(n)), "Z"), ("peek", 3, n, "X")],
+        [("measure_all", list(range(n)), "Y")],
+    ]
+    for k, tail in enumerate(tails):
+        ops = workloads.random_layered(n, 5, 70 + k)
+        prog = [("gate",) + tuple(o) for o in ops] + tail
+        wl.append((n, 2 * n, prog, 500 + k))
+    batch, circs = run_batch(wl, "c128", budget=10 << 20)
+    assert batch.stats()["waves"] >= 3
+    for (n_, nc, prog, s), c in zip(wl, circs):
+        solo = make(n_, nc, prog, s, "c128")
+        solo.execute(SHOTS)
+        got = np.array(c.cstate(), dtype=np.uint64)
+        assert np.array_equal(got, np.array(solo.cstate(), dtype=np.uint64)), s
+        r = make_ref(n_, nc, prog)
+        r.execute(SHOTS, s)
+        assert np.array_equal(got, r.cstate()), s
 
 
 def test_batch_same_circuit_many_seeds():
