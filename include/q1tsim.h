@@ -79,6 +79,18 @@ int q1t_circuit_set_backend(circuit_t* ptr, int precision, int fusion_k, int til
  * parameters re-read; graph_stats counts over the current state. */
 int q1t_circuit_set_graph(circuit_t* ptr, int on);
 int q1t_circuit_graph_stats(circuit_t* ptr, uint64_t* captures, uint64_t* replays);
+/* Build-added, no counterpart in the reference: sweep kernels specialised
+ * to each sweep program of the circuit's state (q1t_gvs_set_jit). */
+int q1t_circuit_set_jit(circuit_t* ptr, int on);
+/* Build-added: plan a gate-only circuit on the host and compile the kernels
+ * specialised to its sweeps with hipRTC (no GPU needed); precision as
+ * q1t_circuit_set_backend, fusion_k > 0 greedy fusion.  With out_dir, each
+ * code object and source is written there as sweep_<i>.co / .hip. */
+int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int fusion_k, const char* out_dir,
+                            size_t* nkernels, double* seconds);
+/* Build-added: specialised kernels compiled in this process, launches of
+ * already loaded ones, and host seconds spent compiling. */
+int q1t_jit_stats(uint64_t* compiled, uint64_t* hits, double* compile_s);
 /* seed of the built-in xoshiro256** stream used by execute/reexecute */
 int q1t_circuit_seed(circuit_t* ptr, uint64_t seed);
 /* initial state for execute: 2^n interleaved amplitudes, or a seeded N(0,1) state */

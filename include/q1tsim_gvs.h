@@ -140,6 +140,14 @@ int q1t_gvs_set_plan_search(q1t_gvs* s, int mode);
  * replay.  Off while profiling or recording; a flush with a 3+-target
  * single-gate kernel runs eagerly.  Results equal the eager path's. */
 int q1t_gvs_set_graph(q1t_gvs* s, int on);
+/* Build-added, no counterpart in the reference: sweep kernels specialised at
+ * run time (on = 1).  Each sweep program's structure (op types, register
+ * slots, controls, phases) becomes straight-line code compiled with hipRTC
+ * on the first flush that needs it and cached per process; the gate
+ * matrices are still read from the uploaded program, so one kernel serves
+ * every program of the same structure.  Same results as the interpreter
+ * kernels (the same op bodies). */
+int q1t_gvs_set_jit(q1t_gvs* s, int on);
 /* Build-added: graphs captured and replays launched over the state's life. */
 int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* replays);
 /* Build-added: tile geometry of the sweeps (M tile bits, R register bits,

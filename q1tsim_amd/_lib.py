@@ -84,6 +84,9 @@ SIGNATURES = {
     "q1t_circuit_seed": (_i32, [_vp, _u64]),
     "q1t_circuit_set_graph": (_i32, [_vp, _i32]),
     "q1t_circuit_graph_stats": (_i32, [_vp, _pu64, _pu64]),
+    "q1t_circuit_set_jit": (_i32, [_vp, _i32]),
+    "q1t_circuit_jit_compile": (_i32, [_vp, _i32, _i32, ctypes.c_char_p, _ps, _pd]),
+    "q1t_jit_stats": (_i32, [_pu64, _pu64, _pd]),
     "q1t_circuit_set_initial_amplitudes": (_i32, [_vp, _pd, _sz]),
     "q1t_circuit_set_initial_random": (_i32, [_vp, _u64]),
     "q1t_circuit_barrier": (_i32, [_vp, _ps, _sz]),
@@ -141,6 +144,7 @@ SIGNATURES = {
     "q1t_gvs_set_plan": (_i32, [_vp, _i32, _i32]),
     "q1t_gvs_set_plan_search": (_i32, [_vp, _i32]),
     "q1t_gvs_set_graph": (_i32, [_vp, _i32]),
+    "q1t_gvs_set_jit": (_i32, [_vp, _i32]),
     "q1t_gvs_graph_stats": (_i32, [_vp, _pu64, _pu64]),
     "q1t_gvs_set_geometry": (_i32, [_vp, _i32, _i32, _i32]),
     "q1t_gvs_last_stats": (_i32, [_vp, ctypes.POINTER(Stats)]),

@@ -9,6 +9,7 @@
 #include "batch.h"
 #include "circuit.h"
 #include "costmodel.h"
+#include "jit.h"
 #include "sweep.h"
 
 using namespace q1t;
@@ -265,6 +266,76 @@ extern "C" int q1t_circuit_set_graph(circuit_t* ptr, int on)
     return guard([&] {
         if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "graph mode must be 0 or 1");
         ptr->c.set_graph(on != 0);
+    });
+}
+
+extern "C" int q1t_circuit_set_jit(circuit_t* ptr, int on)
+{
+    NEED(ptr);
+    return guard([&] {
+        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0 or 1");
+        ptr->c.set_jit(on != 0);
+    });
+}
+
+// Plan a gate-only circuit on the host and compile the kernels specialised
+// to its sweeps (no GPU needed); with out_dir, each code object and source
+// is written there as sweep_<i>.co / sweep_<i>.hip.
+extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int fusion_k, const char* out_dir,
+                                       size_t* nkernels, double* seconds)
+{
+    NEED(ptr);
+    NEED(nkernels);
+    NEED(seconds);
+    return guard([&] {
+        if (precision != PREC_C128 && precision != PREC_C64) throw Error(ERR_INVALID_ARGUMENT, "unknown precision");
+        if (fusion_k < 0 || fusion_k > 6) throw Error(ERR_INVALID_ARGUMENT, "fusion_k must be 0..6");
+        const size_t n = ptr->c.nr_qbits();
+        std::vector<PendingGate> q;
+        for (const CircuitOp& op : ptr->c.ops()) {
+            if (op.type != OpType::Gate) throw Error(ERR_INVALID_ARGUMENT, "jit compile: the circuit holds non-gate ops");
+            std::vector<double> pv;
+            for (const Param& p : op.params) pv.push_back(p.get());
+            const std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+            PendingGate pg;
+            pg.op = analyze_gate(m.data(), op.bits.size(), op.bits.data(), n);
+            pg.matrix = m;
+            for (size_t b : op.bits) pg.qbits.push_back(int(n - 1 - b));
+            q.push_back(std::move(pg));
+        }
+        const std::vector<std::string> srcs = plan_jit_sources(q, int(n), Precision(precision), fusion_k, true);
+        double total = 0.0;
+        for (size_t i = 0; i < srcs.size(); ++i) {
+            double sec = 0.0;
+            const std::vector<char> code = jit_compile(srcs[i], &sec);
+            total += sec;
+            if (out_dir) {
+                const std::string base = std::string(out_dir) + "/sweep_" + std::to_string(i);
+                FILE* f = std::fopen((base + ".co").c_str(), "wb");
+                if (!f) throw Error(ERR_INVALID_ARGUMENT, "cannot write " + base + ".co");
+                std::fwrite(code.data(), 1, code.size(), f);
+                std::fclose(f);
+                f = std::fopen((base + ".hip").c_str(), "wb");
+                if (!f) throw Error(ERR_INVALID_ARGUMENT, "cannot write " + base + ".hip");
+                std::fwrite(srcs[i].data(), 1, srcs[i].size(), f);
+                std::fclose(f);
+            }
+        }
+        *nkernels = srcs.size();
+        *seconds = total;
+    });
+}
+
+extern "C" int q1t_jit_stats(uint64_t* compiled, uint64_t* hits, double* compile_s)
+{
+    NEED(compiled);
+    NEED(hits);
+    NEED(compile_s);
+    return guard([&] {
+        const JitStats s = JitCache::get().stats();
+        *compiled = s.compiled;
+        *hits = s.hits;
+        *compile_s = s.compile_s;
     });
 }
 
@@ -699,6 +770,15 @@ extern "C" int q1t_gvs_set_plan_search(q1t_gvs* s, int mode)
     return guard([&] {
         if (mode < -1 || mode > 1) throw Error(ERR_INVALID_ARGUMENT, "plan search mode must be -1, 0 or 1");
         s->s.set_plan_search(mode);
+    });
+}
+
+extern "C" int q1t_gvs_set_jit(q1t_gvs* s, int on)
+{
+    NEED(s);
+    return guard([&] {
+        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0 or 1");
+        s->s.set_jit(on != 0);
     });
 }
 

@@ -121,6 +121,7 @@ void Circuit::start(size_t nr_shots, std::shared_ptr<SharedStream> stream)
     state_->set_fusion(fusion_k_);
     state_->set_tiling(tiling_);
     state_->set_graph(graph_);
+    state_->set_jit(jit_);
     if (init_mode_ == 1) state_->set_amplitudes(init_amps_.data());
     if (init_mode_ == 2) state_->set_random_normal(init_seed_);
     c_state_.assign(nr_shots, 0);

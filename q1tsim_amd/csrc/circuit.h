@@ -87,6 +87,11 @@ public:
     void set_backend(Precision prec, int fusion_k, bool tiling);
     // flush graphs of the state (GpuVectorState::set_graph); a reexecute
     // keeps the state, so its flushes can replay the previous execute's
+    void set_jit(bool on)
+    {
+        jit_ = on;
+        if (state_) state_->set_jit(on);
+    }
     void set_graph(bool on)
     {
         graph_ = on;
@@ -112,6 +117,7 @@ private:
     int fusion_k_ = 0;
     bool tiling_ = true;
     bool graph_ = false;
+    bool jit_ = false;
     int init_mode_ = 0;            // 0: |0..0>, 1: amplitudes, 2: random normal
     std::vector<cdouble> init_amps_;
     uint64_t init_seed_ = 0;

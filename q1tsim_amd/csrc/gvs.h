@@ -237,6 +237,12 @@ public:
         uint64_t last_use = 0;
     };
     static constexpr size_t kMaxFlushGraphs = 8;
+    // Sweep kernels specialised at run time to each sweep program (off by
+    // default): compiled with hipRTC on the first flush that needs them,
+    // cached per process by program structure.
+    void set_jit(bool on) { jit_on_ = on; }
+    bool jit_enabled() const { return jit_on_; }
+
     void set_graph(bool on);
     bool graph_enabled() const { return graph_on_ && !profiling_ && !recording_; }
     // the graph captured for this signature, or null
@@ -292,6 +298,7 @@ private:
     std::vector<SweepSample> samples_;
     size_t samples_done_ = 0;             // samples whose time is known
 
+    bool jit_on_ = false;
     bool graph_on_ = false;
     std::vector<FlushGraph> fgraphs_;
     uint64_t graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;

@@ -1,9 +1,18 @@
 // Device helpers and the per-gate / measurement kernels (gfx950, wave64).
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#else
+// hipRTC: fixed-width types of its built-in runtime header
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint8_t uint8_t;
+#endif
 
 namespace q1t {
 
@@ -32,15 +41,22 @@ template <typename V> __host__ __device__ __forceinline__ V mkc(typename RealOf<
     v.y = im;
     return v;
 }
+// a*b + c rounded once.  The complex products below spell out their fused
+// multiply-adds, so their results do not depend on how the compiler would
+// contract the surrounding code (the sweep kernels are built in two shapes,
+// interpreter and specialised, that must agree bit for bit).
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename V> __device__ __forceinline__ V cmul(V a, V b)
 {
-    return mkc<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return mkc<V>(fmad(a.x, b.x, -(a.y * b.y)), fmad(a.x, b.y, a.y * b.x));
 }
 template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return mkc<V>(a.x + b.x, a.y + b.y); }
 // acc + a*b
 template <typename V> __device__ __forceinline__ V cmac(V acc, V a, V b)
 {
-    return mkc<V>(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+    return mkc<V>(fmad(a.x, b.x, fmad(-a.y, b.y, acc.x)), fmad(a.x, b.y, fmad(a.y, b.x, acc.y)));
 }
 
 // Deposit zero bits of j at the positions in p (ascending): maps a work index

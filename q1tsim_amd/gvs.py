@@ -50,6 +50,14 @@ def copy_bandwidth(nbytes=8 << 30, reps=5):
     return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+def jit_stats():
+    """(kernels compiled, launches of loaded kernels, compile seconds) of
+    the specialised sweep kernels in this process."""
+    c, h, sec = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_double(0.0)
+    check(lib().q1t_jit_stats(ctypes.byref(c), ctypes.byref(h), ctypes.byref(sec)))
+    return int(c.value), int(h.value), float(sec.value)
+
+
 def set_device_cache_limit(nbytes=0):
     """Cap on released device blocks kept for reuse (0: half of device
     memory); lowering it frees the oldest cached blocks now."""
@@ -133,6 +141,11 @@ class GpuVectorState:
         later flushes have the same launch signature (sweep programs are
         uploaded before each replay)."""
         check(lib().q1t_gvs_set_graph(self.h, int(on)))
+
+    def set_jit(self, on=True):
+        """Sweep kernels specialised to each sweep program (compiled with
+        hipRTC on first use, cached per process by program structure)."""
+        check(lib().q1t_gvs_set_jit(self.h, 1 if on else 0))
 
     def graph_stats(self):
         """(graphs captured, replays launched) over the state's life."""

@@ -69,6 +69,31 @@ def qft(nr_qbits):
     return ops
 
 
+_SELF_INVERSE = ("h", "x", "y", "z", "cx", "cy", "cz", "ch", "swap", "ccx", "ccz", "i")
+_ADJOINT = {"s": "sdg", "sdg": "s", "t": "tdg", "tdg": "t", "cs": "csdg", "csdg": "cs", "ct": "ctdg",
+            "ctdg": "ct", "v": "vdg", "vdg": "v", "cv": "cvdg", "cvdg": "cv"}
+_NEGATED = ("rx", "ry", "rz", "u1", "crx", "cry", "crz", "cu1", "ccrx", "ccry", "ccrz")
+
+
+def inverse(ops):
+    """The inverse circuit: the gates in reverse order, each replaced by its
+    adjoint (rotation angles negated; U3(t, p, l)^-1 = U3(-t, -l, -p))."""
+    out = []
+    for name, bits, params in reversed(ops):
+        if name in _SELF_INVERSE:
+            out.append((name, list(bits), []))
+        elif name in _ADJOINT:
+            out.append((_ADJOINT[name], list(bits), []))
+        elif name in _NEGATED:
+            out.append((name, list(bits), [-float(params[0])]))
+        elif name in ("u3", "cu3"):
+            t, p, l = params
+            out.append((name, list(bits), [-float(t), -float(l), -float(p)]))
+        else:
+            raise ValueError("no inverse for gate %r" % (name,))
+    return out
+
+
 def branching(nr_qbits, depth, nr_meas, seed):
     """Layered random circuit (the recipe of `random_layered`) with `nr_meas`
     mid-circuit Z measurements at distinct random layer boundaries.
