@@ -100,6 +100,16 @@ int q1t_gvs_apply_conditional_gate(q1t_gvs* s, const uint8_t* control, size_t nc
 int q1t_gvs_marginal_w0(q1t_gvs* s, size_t qbit, double* w0s, size_t len);
 int q1t_gvs_measure_with_n0(q1t_gvs* s, size_t qbit, size_t cbit, const size_t* n0s, const double* w0s,
                             size_t ncols, uint64_t* res, size_t res_len, int collapse);
+/* No counterpart in the reference.  q1t_gvs_marginal_w0 for callers that
+ * cannot allocate device memory (code under graph capture, tight serving
+ * loops): the call makes no hipMalloc/hipFree.  Its partial sums live in a
+ * fixed 516 KiB device array per GPU, held until the process ends; at most
+ * 256 branch columns (a larger state is rejected with
+ * Q1T_ERR_INVALID_ARGUMENT), no pending gates (flush first), and the device
+ * current at the call must be the state's.  Results are bit-identical to
+ * q1t_gvs_marginal_w0.  Calls that overlap on one GPU share the array and
+ * race: serialise them. */
+int q1t_gvs_marginal_w0_noalloc(q1t_gvs* s, size_t qbit, double* w0s, size_t len);
 /* QuState::measure_into (qustate.rs:35, vectorstate.rs:237-329). */
 int q1t_gvs_measure_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len, const q1t_rng* rng);
 /* QuState::peek_into (qustate.rs:62, vectorstate.rs:346-393). */

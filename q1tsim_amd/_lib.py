@@ -131,6 +131,7 @@ SIGNATURES = {
     "q1t_gvs_apply_unary_gate_all": (_i32, [_vp, _pd, _cs]),
     "q1t_gvs_apply_conditional_gate": (_i32, [_vp, _pu8, _sz, _pd, _sz, _ps, _sz, _cs]),
     "q1t_gvs_marginal_w0": (_i32, [_vp, _sz, _pd, _sz]),
+    "q1t_gvs_marginal_w0_noalloc": (_i32, [_vp, _sz, _pd, _sz]),
     "q1t_gvs_measure_with_n0": (_i32, [_vp, _sz, _sz, _ps, _pd, _sz, _pu64, _sz, _i32]),
     "q1t_gvs_measure_into": (_i32, [_vp, _sz, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),
     "q1t_gvs_peek_into": (_i32, [_vp, _sz, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),

@@ -656,6 +656,13 @@ extern "C" int q1t_gvs_marginal_w0(q1t_gvs* s, size_t qbit, double* w0s, size_t 
     });
 }
 
+extern "C" int q1t_gvs_marginal_w0_noalloc(q1t_gvs* s, size_t qbit, double* w0s, size_t len)
+{
+    NEED(s);
+    NEED(w0s);
+    return guard([&] { s->s.marginal_w0_noalloc(qbit, w0s, len); });
+}
+
 extern "C" int q1t_gvs_measure_with_n0(q1t_gvs* s, size_t qbit, size_t cbit, const size_t* n0s, const double* w0s,
                                        size_t ncols, uint64_t* res, size_t res_len, int collapse)
 {

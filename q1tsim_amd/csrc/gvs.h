@@ -144,6 +144,11 @@ public:
     void apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
                                 size_t nr_affected, const size_t* qbits, size_t nbits, const std::string& desc);
     std::vector<double> marginal_w0(size_t qbit);
+    // marginal_w0 without any device allocation: partial sums in a fixed
+    // file-scope device array (at most kNoAllocMaxCols columns), summed on
+    // the device in the host's order (bit-identical results).  Needs no
+    // pending gates.  Calls that overlap on one GPU race on that array.
+    void marginal_w0_noalloc(size_t qbit, double* w0s, size_t len);
     // marginal_w0 in two steps (batches): enqueue the reduction and its copy
     // into `host` (pinned, >= marginal_partials() doubles); once the stream
     // has synchronised, collect gives exactly what marginal_w0 gives.
@@ -273,6 +278,7 @@ private:
     void* amp_ = nullptr;                 // column-major: column c at amp_ + c * 2^n elements
     size_t amp_cols_ = 0;                 // allocated columns
     hipStream_t stream_ = nullptr;
+    int device_ = 0;                      // the device current at construction
     std::shared_ptr<SharedStream> shared_;   // set: stream_ belongs to it
     std::vector<PendingGate> queue_;
     RunStats stats_;

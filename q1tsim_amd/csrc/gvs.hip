@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "kernels.h"
 #include "sweep.h"
@@ -575,6 +576,7 @@ GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw Error(ERR_HIP, "No HIP device available for the GPU statevector");
+    HIP_OK(hipGetDevice(&device_));
     if (shared) {
         shared_ = std::move(shared);
         stream_ = shared_->s;
@@ -1124,6 +1126,83 @@ void GpuVectorState::marginal_w0_async(size_t qbit, double* host)
 std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const double* host) const
 {
     return w0_from_partials(qbit, host);
+}
+
+// ------------------------------------------- allocation-free marginal w0
+
+// The scratch of marginal_w0_noalloc: kNormBlocks partial sums per column
+// (columns at stride kNormBlocks), then one w0 per column.  516 KiB; each GPU
+// the library runs on holds its own copy until the process ends.  Every call
+// overwrites the entries it reads, so nothing needs clearing between calls.
+static constexpr size_t kNoAllocMaxCols = 256;
+static constexpr size_t kNoAllocW0 = kNoAllocMaxCols * kNormBlocks;
+static __device__ __align__(256) double scratch[kNoAllocW0 + kNoAllocMaxCols];
+
+// w0 of each column: the partials summed one after another from 0.0, the
+// order w0_from_partials uses on the host.
+__global__ void __launch_bounds__(kBlock) k_sum_partials(const double* __restrict__ part, unsigned nblocks,
+                                                         unsigned ncols, double* __restrict__ out)
+{
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (unsigned b = 0; b < nblocks; ++b) s += part[size_t(c) * nblocks + b];
+    out[c] = s;
+}
+
+// The device address of `scratch` on each device, looked up once per device
+// with that device current.  The array's name is not a device address in
+// host code; only these pointers are passed to kernels and copies.
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static double* noalloc_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) throw Error(ERR_HIP, "Device index out of range: " + std::to_string(dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            throw Error(ERR_HIP, std::string("hipGetSymbolAddress of the marginal scratch failed: ") +
+                                     hipGetErrorString(e));
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
+{
+    check_qbit(qbit);
+    const size_t nc = counts_.size();
+    if (len < nc) throw err_not_enough_space(len, nc);
+    if (!queue_.empty())
+        throw Error(ERR_INVALID_ARGUMENT, "Gates are pending on the state: flush it before the allocation-free marginal");
+    if (basis_) {
+        const int p = int(nr_bits_ - 1 - qbit);
+        for (size_t c = 0; c < nc; ++c) w0s[c] = ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+        return;
+    }
+    if (nc > kNoAllocMaxCols)
+        throw Error(ERR_INVALID_ARGUMENT, "The allocation-free marginal takes at most " +
+                                              std::to_string(kNoAllocMaxCols) + " columns; the state has " +
+                                              std::to_string(nc));
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != device_)
+        throw Error(ERR_INVALID_ARGUMENT, "The state lives on device " + std::to_string(device_) +
+                                              ", but device " + std::to_string(dev) + " is current");
+    double* part = noalloc_scratch(dev);
+    double* w = part + kNoAllocW0;
+    launch_marginal(qbit, part);   // one launch: nc <= 65535 columns
+    hipLaunchKernelGGL(k_sum_partials, dim3(unsigned((nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream_, part,
+                       kNormBlocks, unsigned(nc), w);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(w0s, w, nc * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,

@@ -198,6 +198,13 @@ class GpuVectorState:
         check(lib().q1t_gvs_marginal_w0(self.h, qbit, _dptr(out), len(out)))
         return out
 
+    def marginal_w0_noalloc(self, qbit):
+        """marginal_w0 through the allocation-free entry point (at most 256
+        columns, no pending gates: call flush() first)."""
+        out = np.zeros(self.ncols, dtype=np.float64)
+        check(lib().q1t_gvs_marginal_w0_noalloc(self.h, qbit, _dptr(out), len(out)))
+        return out
+
     def measure_into(self, qbit, cbit, res, rng, collapse=True):
         f = lib().q1t_gvs_measure_into if collapse else lib().q1t_gvs_peek_into
         check(f(self.h, qbit, cbit, _u64ptr(res), len(res), ctypes.byref(rng.rng)))
