@@ -6,13 +6,18 @@ CX), statevector resident in HBM.  A step is one application of the whole
 circuit to the resident state.
 
   * plain  : complex128 state, per-gate FP64 arithmetic, gates grouped into
-             LDS/register tile sweeps (the reported `value`, in HBM GB/s)
+             LDS/register tile sweeps, each sweep run by the kernel
+             specialised to its program at run time (the reported `value`,
+             in HBM GB/s; compiled during the warm-up steps); the same
+             circuit on the interpreter kernels is reported beside it
   * fused  : complex64 state, fusion width 4: per sweep the learned cost
              model (trained in this run on sweeps the simulator records
              itself) picks register phases or 3-/4-qubit FP32-MFMA blocks;
              its time, speedup over plain and infidelity / max-abs error
              against the plain complex128 state are reported, beside greedy
-             fixed-width fusion and the per-gate complex64 path.
+             fixed-width fusion for k = 3..6 and the per-gate complex64
+             path; `roofline_fused` puts greedy 4-qubit fusion's kernel
+             against the FP32 MFMA peak.
 
 Also reported: the 30-qubit QFT on a random normal state, FP64 sampling of
 2^16 runs (measure_all, peek semantics), a batch of 4096 circuits with
@@ -47,6 +52,9 @@ def apply_circuit(state, ops):
 
 
 def timed_steps(state, ops, steps, warmup, profile=True):
+    """Median-able step times of the whole circuit on the resident state.
+    The warm-up steps also compile the specialised sweep kernels when the
+    state uses them (the first flush of a program structure compiles it)."""
     for _ in range(warmup):
         apply_circuit(state, ops)
     state.set_profiling(profile)
@@ -98,34 +106,65 @@ def profiled_traffic(kernel, n):
     return None if vals is None else float(vals["traffic_bytes_per_launch"])
 
 
-def cpu_baseline(ops_full, n_full, budget_s=20.0):
-    """The oracle (C restatement of VectorState, 1 core) on the first gates of
-    the same circuit at n=26 (1 GiB state), extrapolated linearly in gates
-    and in 2^n to the full 30-qubit circuit."""
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(ops_full, n_full, n=22, layers=2, reps=3):
+    """The oracle (C restatement of VectorState, 1 core, the reference's data
+    movement: per-block temporaries, table-based permutations) timed as the
+    CPU-baseline protocol asks: the gate stream and the sampling separately,
+    each the median of `reps` runs after one warm-up.  Gate stream: the first
+    `layers` layers of the same layered recipe at n qubits; per-gate times
+    scale with 2^n, so the 30-qubit circuit time is that median x (gates of
+    the full circuit / gates timed) x 2^(30-n), stated in `sample`.
+    Sampling: peek_all of 2^16 runs at n qubits (the reference's
+    measure_all collapse at 30 qubits would need ~1 PB)."""
     from oracle import oracle as orc
     from q1tsim_amd import workloads
-    n = 26
-    ops = workloads.random_layered(n, 20, 1)
-    st = orc.RefState(n, 1)
-    t0 = time.perf_counter()
-    done = 0
-    for name, bits, params in ops:
-        st.apply_gate(name, bits, params)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    ops = [op for op in workloads.random_layered(n, layers, 1)]
+    gate_t = []
+    for rep in range(reps + 1):
+        st = orc.RefState(n, 1)
+        t0 = time.perf_counter()
+        for name, bits, params in ops:
+            st.apply_gate(name, bits, params)
+        if rep:
+            gate_t.append(time.perf_counter() - t0)
+    samp_t = []
+    res = np.zeros(1 << 16, np.uint64)
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        st.measure_all_into(list(range(n)), res, orc.Rng(rep + 1), collapse=False)
+        if rep:
+            samp_t.append(time.perf_counter() - t0)
+    g_s = float(np.median(gate_t))
+    s_s = float(np.median(samp_t))
     scale_n = float(1 << (n_full - n))
-    est_circuit_s = dt / done * len(ops_full) * scale_n
+    est_circuit_s = g_s * len(ops_full) / len(ops) * scale_n
     S = float(16 << n_full)
     unf = workloads.unfused_bytes(ops_full, S)
+    model, nproc = host_cpu()
     return {"value": round(unf / est_circuit_s / 1e9, 3), "unit": "GB/s (unfused-equivalent)", "cores": 1,
             "kind": "port",
-            "sample": "oracle (C restatement of q1tsim VectorState, FP64, 1 thread) applying the first %d gates "
-                      "of the layered recipe at n=%d in %.1f s; extrapolated x%d gates-ratio and x%d for 2^30 "
-                      "-> %.1f s per 30-qubit circuit" % (done, n, dt, len(ops_full) // max(done, 1), int(scale_n),
-                                                          est_circuit_s),
-            "circuit_seconds_est": round(est_circuit_s, 2)}
+            "sample": "oracle (C restatement of q1tsim VectorState, FP64, 1 thread): gate stream = first %d "
+                      "layers (%d gates) of the layered recipe at n=%d, median of %d runs after 1 warm-up = %.3f s; "
+                      "extrapolated x%.2f (890/%d gates) x%d (2^30/2^%d) -> %.1f s per 30-qubit circuit. "
+                      "Sampling: peek_all of 2^16 runs at n=%d, median %.4f s"
+                      % (layers, len(ops), n, reps, g_s, len(ops_full) / len(ops), len(ops), int(scale_n), n,
+                         est_circuit_s, n, s_s),
+            "gate_stream_s": round(g_s, 4), "gate_stream_runs_s": [round(t, 4) for t in gate_t],
+            "sampling_65536_s": round(s_s, 5), "circuit_seconds_est": round(est_circuit_s, 2),
+            "host_cpu": model, "host_nproc": nproc}
 
 
 def batch_extra(count, seed=2024):
@@ -205,6 +244,9 @@ def learned_extra(cm, info, n=32, depth=40, seed=5):
     from q1tsim_amd.gvs import GpuVectorState, compare
     ops = workloads.random_pairs(n, depth, seed)
     ref = GpuVectorState(n, 1, "c128")
+    ref.set_jit(True)
+    apply_circuit(ref, ops)   # warm-up (compiles the specialised kernels)
+    ref.reset_all()
     t0 = time.perf_counter()
     apply_circuit(ref, ops)
     c128_s = time.perf_counter() - t0
@@ -214,7 +256,8 @@ def learned_extra(cm, info, n=32, depth=40, seed=5):
         s = GpuVectorState(n, 1, "c64")
         s.set_plan(4, True)
         s.set_cost_model(cm, mode, seed)
-        apply_circuit(s, ops)   # warm-up
+        s.set_jit(True)
+        apply_circuit(s, ops)   # warm-up (compiles the specialised kernels)
         s.reset_all()
         t0 = time.perf_counter()
         apply_circuit(s, ops)
@@ -231,7 +274,7 @@ def learned_extra(cm, info, n=32, depth=40, seed=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--depth", type=int, default=20)
@@ -271,11 +314,17 @@ def main():
     n = args.n
     ops = workloads.random_layered(n, args.depth, 1)
     S128 = float(16 << n)
+    from q1tsim_amd.gvs import jit_stats
 
-    # plain complex128
+    # plain complex128 (the reported `value`): per-gate FP64 arithmetic in
+    # tile sweeps, each sweep run by the kernel specialised to its program
+    # (compiled during the warm-up steps)
+    jit0 = jit_stats()
     plain = GpuVectorState(n, 1, "c128")
     plain.set_plan(0, True)
-    t_plain, sw_plain, bytes_plain, prof_plain = timed_steps(plain, ops, args.steps, args.warmup)
+    plain.set_jit(True)
+    t_plain, sw_plain, bytes_plain, prof_plain = timed_steps(plain, ops, args.steps, max(args.warmup, 1))
+    jit1 = jit_stats()
     ms_plain = 1e3 * float(np.median(t_plain))
     kname, roof = roofline_of(prof_plain)
     roof["traffic"] = profiled_traffic(kname, n)
@@ -290,29 +339,56 @@ def main():
         issue = float(valu["valu_issue_ms_per_launch"])
         roof["valu_issue"] = {"ms_per_launch": round(issue, 4), "frac_of_launch": round(issue / roof["avg_ms"], 4),
                               "source": "profiles/%s_layered%d_valu.txt (rocprofv3 SQ_INSTS_VALU)" % (kname, n)}
+    jit_info = {"kernels_compiled": jit1[0] - jit0[0], "compile_cpu_s": round(jit1[2] - jit0[2], 2),
+                "note": "compiled in the warm-up steps; the timed steps reuse them"}
+    # the same circuit on the interpreter kernels (no run-time compilation)
+    interp = GpuVectorState(n, 1, "c128")
+    interp.set_plan(0, True)
+    t_int, _, _, prof_int = timed_steps(interp, ops, 2, 1)
+    infid_ji, maxabs_ji = compare(plain, interp)
+    del interp
 
-    # complex64: the fused path as the learned planner runs it (per sweep:
-    # register phases, 3- or 4-qubit MFMA blocks), beside greedy fixed-width
-    # fusion and the per-gate c64 path
+    # complex64: the per-gate path, greedy fixed-width fusion into k-qubit
+    # FP32-MFMA blocks for k = 3..6, and the learned planner's choice per
+    # sweep (register phases or 3-/4-qubit blocks), all on specialised
+    # kernels; errors against the complex128 state
     cm, cm_info = train_cost_model()
     c64 = {}
-    for name, fk, mode in (("learned", args.fusion_k, 1), ("greedy_fused", args.fusion_k, 0), ("per_gate", 0, 0)):
+    legs = [("per_gate", 0, 0)] + [("greedy_k%d" % k, k, 0) for k in (3, 4, 5, 6)] + [("learned", args.fusion_k, 1)]
+    for name, fk, mode in legs:
         st64 = GpuVectorState(n, 1, "c64")
         st64.set_plan(fk, True)
         st64.set_cost_model(cm, mode, 5)
-        t, sw, nb, prof = timed_steps(st64, ops, args.steps, args.warmup)
+        st64.set_jit(True)
+        t, sw, nb, prof = timed_steps(st64, ops, args.steps, max(args.warmup, 1))
         infid, maxabs = compare(plain, st64)
-        c64[name] = {"ms": round(1e3 * float(np.median(t)), 3), "sweeps": sw, "hbm_bytes": nb,
-                     "blocks": st64.stats()["blocks"], "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs,
+        stt = st64.stats()
+        c64[name] = {"ms": round(1e3 * float(np.median(t)), 3), "sweeps": sw, "hbm_bytes": nb, "fusion_k": fk,
+                     "blocks": stt["blocks"], "mfma_flops": stt["mfma_flops"], "infidelity_vs_c128": infid,
+                     "max_abs_vs_c128": maxabs,
                      "kernels": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof.items()}}
         del st64
     ms_fused = c64["learned"]["ms"]
+    # the fused kernel against its own roofline: greedy 4-qubit fusion (a
+    # block in every sweep), MFMA flops per launch over the launch time, and
+    # the same launches' HBM bytes
+    g4 = c64["greedy_k4"]
+    kf = max(g4["kernels"], key=lambda k: g4["kernels"][k]["ms"])
+    lf, msf = g4["kernels"][kf]["launches"], g4["kernels"][kf]["ms"]
+    tflops = g4["mfma_flops"] / lf / (msf / lf * 1e-3) / 1e12
+    roof_fused = {"bound": "mfma", "achieved": round(tflops, 2), "peak": 157.3, "unit": "TFLOP/s",
+                  "frac": round(tflops / 157.3, 4), "traffic": None, "kernel": kf, "workload": "greedy 4-qubit fusion",
+                  "avg_ms": round(msf / lf, 4), "mfma_flops_per_launch": g4["mfma_flops"] / lf,
+                  "hbm_GBps": round(g4["hbm_bytes"] / (msf * 1e-3) / 1e9, 1),
+                  "hbm_frac": round(g4["hbm_bytes"] / (msf * 1e-3) / 1e9 / PEAK_GBPS, 4)}
 
     extra = {}
     if not args.no_extra:
         # QFT on a random normal state + FP64 sampling of 2^16 runs
-        plain.set_random_normal(7)
         q = workloads.qft(n)
+        plain.set_random_normal(7)
+        apply_circuit(plain, q)   # warm-up: compiles the QFT's specialised kernels
+        plain.set_random_normal(7)
         t0 = time.perf_counter()
         apply_circuit(plain, q)
         extra["qft30_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
@@ -357,16 +433,24 @@ def main():
                    "n_qubits": n, "depth": args.depth, "gates": len(ops), "seed": 1},
         "plain_c128": {"ms": round(ms_plain, 3), "sweeps": sw_plain, "hbm_bytes": bytes_plain,
                        "GBps": round(value, 1), "frac_of_peak": round(value / PEAK_GBPS, 4),
-                       "unfused_equivalent_GBps": round(unf / (ms_plain * 1e-3) / 1e9, 1)},
+                       "unfused_equivalent_GBps": round(unf / (ms_plain * 1e-3) / 1e9, 1),
+                       "kernels": "specialised per sweep program (hipRTC)", "jit": jit_info,
+                       "interpreter_ms": round(1e3 * float(np.median(t_int)), 3),
+                       "interpreter_kernels": {k: {"launches": v[0], "ms": round(v[1], 3)}
+                                               for k, v in prof_int.items()},
+                       "interpreter_vs_specialised": {"infidelity": infid_ji, "max_abs": maxabs_ji}},
         "fused_c64": {"ms": ms_fused, "planner": "learned (cost model trained in this run)",
                       "sweeps": c64["learned"]["sweeps"], "blocks": c64["learned"]["blocks"],
                       "GBps": round(c64["learned"]["hbm_bytes"] / (ms_fused * 1e-3) / 1e9, 1),
                       "fusion_k": args.fusion_k, "speedup_vs_plain": round(ms_plain / ms_fused, 3),
+                      "speedup_vs_plain_interpreter": round(1e3 * float(np.median(t_int)) / ms_fused, 3),
                       "infidelity_vs_c128": c64["learned"]["infidelity_vs_c128"],
                       "max_abs_vs_c128": c64["learned"]["max_abs_vs_c128"], "dtype": "f32",
-                      "greedy_fused_ms": c64["greedy_fused"]["ms"], "per_gate_c64_ms": c64["per_gate"]["ms"],
+                      "per_k": {k: {kk: v[kk] for kk in ("ms", "sweeps", "blocks", "mfma_flops", "infidelity_vs_c128",
+                                                         "max_abs_vs_c128")} for k, v in c64.items()},
                       "cost_model": cm_info},
         "roofline": roof,
+        "roofline_fused": roof_fused,
         "kernels_plain": {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in prof_plain.items()},
         "kernels_fused": c64["learned"]["kernels"],
     }

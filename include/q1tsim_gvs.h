@@ -68,6 +68,7 @@ typedef struct {
     uint64_t gates;        /* gates applied */
     uint64_t blocks;       /* fused dense blocks applied (per column) */
     uint64_t phases;       /* in-tile phases (LDS exchanges) over all sweeps */
+    double mfma_flops;     /* FP32 MFMA flops of the fused blocks: 2^(n+k+3) per k-qubit block and column */
 } q1t_stats;
 
 const char* q1t_last_error(void);
@@ -156,7 +157,8 @@ int q1t_gvs_set_graph(q1t_gvs* s, int on);
  * on the first flush that needs it and cached per process; the gate
  * matrices are still read from the uploaded program, so one kernel serves
  * every program of the same structure.  Same results as the interpreter
- * kernels (the same op bodies). */
+ * kernels (the same op bodies).  on = 2: the same kernels with each
+ * workgroup's next tile loaded into registers while the current one runs. */
 int q1t_gvs_set_jit(q1t_gvs* s, int on);
 /* Build-added: graphs captured and replays launched over the state's life. */
 int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* replays);

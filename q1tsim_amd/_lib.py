@@ -28,7 +28,7 @@ class Result(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("gate_seconds", ctypes.c_double), ("bytes", ctypes.c_double), ("launches", ctypes.c_uint64),
                 ("sweeps", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("blocks", ctypes.c_uint64),
-                ("phases", ctypes.c_uint64)]
+                ("phases", ctypes.c_uint64), ("mfma_flops", ctypes.c_double)]
 
 
 class KernelProfile(ctypes.Structure):
@@ -85,7 +85,7 @@ SIGNATURES = {
     "q1t_circuit_set_graph": (_i32, [_vp, _i32]),
     "q1t_circuit_graph_stats": (_i32, [_vp, _pu64, _pu64]),
     "q1t_circuit_set_jit": (_i32, [_vp, _i32]),
-    "q1t_circuit_jit_compile": (_i32, [_vp, _i32, _i32, ctypes.c_char_p, _ps, _pd]),
+    "q1t_circuit_jit_compile": (_i32, [_vp, _i32, _i32, _i32, ctypes.c_char_p, _ps, _pd]),
     "q1t_jit_stats": (_i32, [_pu64, _pu64, _pd]),
     "q1t_circuit_set_initial_amplitudes": (_i32, [_vp, _pd, _sz]),
     "q1t_circuit_set_initial_random": (_i32, [_vp, _u64]),

@@ -273,16 +273,16 @@ extern "C" int q1t_circuit_set_jit(circuit_t* ptr, int on)
 {
     NEED(ptr);
     return guard([&] {
-        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0 or 1");
-        ptr->c.set_jit(on != 0);
+        if (on < 0 || on > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+        ptr->c.set_jit(on);
     });
 }
 
 // Plan a gate-only circuit on the host and compile the kernels specialised
 // to its sweeps (no GPU needed); with out_dir, each code object and source
 // is written there as sweep_<i>.co / sweep_<i>.hip.
-extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int fusion_k, const char* out_dir,
-                                       size_t* nkernels, double* seconds)
+extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int fusion_k, int mode,
+                                       const char* out_dir, size_t* nkernels, double* seconds)
 {
     NEED(ptr);
     NEED(nkernels);
@@ -290,6 +290,7 @@ extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int 
     return guard([&] {
         if (precision != PREC_C128 && precision != PREC_C64) throw Error(ERR_INVALID_ARGUMENT, "unknown precision");
         if (fusion_k < 0 || fusion_k > 6) throw Error(ERR_INVALID_ARGUMENT, "fusion_k must be 0..6");
+        if (mode != 1 && mode != 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 1 or 2");
         const size_t n = ptr->c.nr_qbits();
         std::vector<PendingGate> q;
         for (const CircuitOp& op : ptr->c.ops()) {
@@ -303,7 +304,7 @@ extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int 
             for (size_t b : op.bits) pg.qbits.push_back(int(n - 1 - b));
             q.push_back(std::move(pg));
         }
-        const std::vector<std::string> srcs = plan_jit_sources(q, int(n), Precision(precision), fusion_k, true);
+        const std::vector<std::string> srcs = plan_jit_sources(q, int(n), Precision(precision), fusion_k, true, mode == 2);
         double total = 0.0;
         for (size_t i = 0; i < srcs.size(); ++i) {
             double sec = 0.0;
@@ -446,7 +447,7 @@ extern "C" int q1t_circuit_last_stats(const circuit_t* ptr, q1t_stats* out)
     NEED(ptr);
     NEED(out);
     const RunStats& r = ptr->c.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases, r.mfma_flops};
     return 0;
 }
 
@@ -495,7 +496,7 @@ extern "C" int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out)
     NEED(out);
     const BatchStats& s = b->b.stats();
     const RunStats& r = s.gates;
-    out->gates = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
+    out->gates = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases, r.mfma_flops};
     out->waves = s.waves;
     out->rounds = s.rounds;
     out->sync_ops = s.sync_ops;
@@ -784,8 +785,8 @@ extern "C" int q1t_gvs_set_jit(q1t_gvs* s, int on)
 {
     NEED(s);
     return guard([&] {
-        if (on != 0 && on != 1) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0 or 1");
-        s->s.set_jit(on != 0);
+        if (on < 0 || on > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+        s->s.set_jit(on);
     });
 }
 
@@ -829,7 +830,7 @@ extern "C" int q1t_gvs_last_stats(const q1t_gvs* s, q1t_stats* out)
     NEED(s);
     NEED(out);
     const RunStats& r = s->s.last_stats();
-    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases};
+    *out = q1t_stats{r.gate_seconds, r.bytes, r.launches, r.sweeps, r.gates, r.blocks, r.phases, r.mfma_flops};
     return 0;
 }
 

@@ -87,10 +87,10 @@ public:
     void set_backend(Precision prec, int fusion_k, bool tiling);
     // flush graphs of the state (GpuVectorState::set_graph); a reexecute
     // keeps the state, so its flushes can replay the previous execute's
-    void set_jit(bool on)
+    void set_jit(int mode)
     {
-        jit_ = on;
-        if (state_) state_->set_jit(on);
+        jit_ = mode;
+        if (state_) state_->set_jit(mode);
     }
     void set_graph(bool on)
     {
@@ -117,7 +117,7 @@ private:
     int fusion_k_ = 0;
     bool tiling_ = true;
     bool graph_ = false;
-    bool jit_ = false;
+    int jit_ = 0;
     int init_mode_ = 0;            // 0: |0..0>, 1: amplitudes, 2: random normal
     std::vector<cdouble> init_amps_;
     uint64_t init_seed_ = 0;

@@ -23,6 +23,7 @@ struct RunStats {
     uint64_t gates = 0;           // gates applied
     uint64_t blocks = 0;          // fused dense blocks applied per column
     uint64_t phases = 0;          // in-tile phases over all sweeps
+    double mfma_flops = 0.0;      // FP32 MFMA flops of the fused blocks: 2^(n+k+3) per k-qubit block and column
 };
 
 // Tile geometry of the sweeps: M local bits per tile, R of them in registers
@@ -245,8 +246,13 @@ public:
     // Sweep kernels specialised at run time to each sweep program (off by
     // default): compiled with hipRTC on the first flush that needs them,
     // cached per process by program structure.
-    void set_jit(bool on) { jit_on_ = on; }
-    bool jit_enabled() const { return jit_on_; }
+    // With specialised kernels and no geometry set explicitly, the tile
+    // shape is the one measured best for them (tile_geometry_specialised).
+    // mode 0: interpreter kernels; 1: specialised kernels; 2: specialised
+    // kernels with the next tile prefetched into registers.
+    void set_jit(int mode);
+    bool jit_enabled() const { return jit_mode_ != 0; }
+    bool jit_prefetch() const { return jit_mode_ == 2; }
 
     void set_graph(bool on);
     bool graph_enabled() const { return graph_on_ && !profiling_ && !recording_; }
@@ -304,7 +310,8 @@ private:
     std::vector<SweepSample> samples_;
     size_t samples_done_ = 0;             // samples whose time is known
 
-    bool jit_on_ = false;
+    int jit_mode_ = 0;
+    bool geom_set_ = false;               // set_geometry was called
     bool graph_on_ = false;
     std::vector<FlushGraph> fgraphs_;
     uint64_t graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;

@@ -606,6 +606,14 @@ void GpuVectorState::set_geometry(const TileGeom& g)
                                               std::to_string(g.R) + " L=" + std::to_string(g.L));
     flush();
     geom_ = g;
+    geom_set_ = true;
+}
+
+void GpuVectorState::set_jit(int mode)
+{
+    if (mode < 0 || mode > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+    jit_mode_ = mode;
+    if (!geom_set_) geom_ = mode ? tile_geometry_specialised(prec_) : tile_geometry(prec_);
 }
 
 GpuVectorState::~GpuVectorState()

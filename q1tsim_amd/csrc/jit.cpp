@@ -138,7 +138,7 @@ static void emit_block_phase(std::ostringstream& o, const SwPhase& ph, int M)
 }
 
 std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, bool c64, int M,
-                             int R, int SW)
+                             int R, int SW, bool prefetch)
 {
     std::ostringstream o;
     const int NT = 1 << (M - R);
@@ -155,6 +155,19 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
     o << "    __shared__ V lds[1 << M];\n";
     o << "    V* a = amp + uint64_t(blockIdx.y) * col_stride;\n";
     o << "    TileAddr<M, R> ta;\n    tile_addr<M, R>(sa, ta);\n";
+    if (prefetch) {
+        // the first tile's share in registers before the loop; every
+        // iteration stores the prefetched share to LDS, then issues the loads
+        // of the next tile (in flight while the phases run)
+        // (scalars, not an array: a loop-carried array of vector structs
+        // is not promoted to registers)
+        const int NR = 1 << R;
+        for (int k = 0; k < NR; ++k) o << "    V pf" << k << ";\n";
+        o << "    if (uint64_t(blockIdx.x) < sa.ntiles) {\n";
+        o << "        const uint64_t b0 = deposit(uint64_t(blockIdx.x), sa.tset);\n";
+        for (int k = 0; k < NR; ++k) o << "        pf" << k << " = a[b0 | ta.g_tid | ta.g_k[" << k << "]];\n";
+        o << "    }\n";
+    }
     o << "    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) {\n";
     // The thread index and the table pointers are made opaque per tile:
     // otherwise every phase's LDS addresses (functions of the thread index
@@ -164,13 +177,46 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
     o << "    const SwOp* ops = sa.ops;\n    asm volatile(\"\" : \"+s\"(ops));\n";
     o << "    const float* blocks = sa.blocks;\n    asm volatile(\"\" : \"+s\"(blocks));\n";
     o << "    const uint64_t base = deposit(tile, sa.tset);\n";
-    o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];\n";
-    o << "    __syncthreads();\n";
-    for (const SwPhase& ph : phases) {
-        if (ph.kind == 1) emit_block_phase(o, ph, M);
-        else emit_register_phase(o, ph, ops, M, R, SW);
+    if (prefetch) {
+        const int NR = 1 << R;
+        for (int k = 0; k < NR; ++k) o << "    lds[swz<SW, M>(tid + " << k << " * NT)] = pf" << k << ";\n";
+        o << "    __syncthreads();\n";
+        o << "    if (tile + gridDim.x < sa.ntiles) {\n";
+        o << "        const uint64_t nb = deposit(tile + gridDim.x, sa.tset);\n";
+        for (int k = 0; k < NR; ++k) o << "        pf" << k << " = a[nb | ta.g_tid | ta.g_k[" << k << "]];\n";
+        o << "    }\n";
+    } else {
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];\n";
+        o << "    __syncthreads();\n";
     }
-    o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];\n";
+    bool scaled = false;
+    for (const SwPhase& ph : phases) {
+        if (ph.kind == 1) {
+            emit_block_phase(o, ph, M);
+        } else if (ph.kind == 2) {
+            // the tile factor: the product of tile_factor() in program order
+            scaled = true;
+            o << "    V fac = mkc<V>(1, 0);\n";
+            for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
+                const SwOp& op = ops[oi];
+                o << "    {\n";
+                if (op.co) o << "    if ((base & " << hex64(op.co) << ") == " << hex64(op.co) << ") {\n";
+                o << "        const SwOp op = load_uniform(ops, " << oi << "u);\n";
+                if (op.type == SW_PHASE) o << "        fac = cmul(fac, opm<V>(op, 0));\n";
+                else
+                    o << "        fac = cmul(fac, sel(((base >> " << op.dglobal
+                      << ") & 1) != 0, opm<V>(op, 1), opm<V>(op, 0)));\n";
+                if (op.co) o << "    }\n";
+                o << "    }\n";
+            }
+        } else {
+            emit_register_phase(o, ph, ops, M, R, SW);
+        }
+    }
+    if (scaled)
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = cmul(fac, lds[swz<SW, M>(tid + k * NT)]);\n";
+    else
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];\n";
     o << "    __syncthreads();\n    }\n}\n";
     return o.str();
 }

@@ -25,8 +25,10 @@ namespace q1t {
 // arguments as k_sweep<V, M, R, SW, *> (amplitudes, column stride,
 // SweepArgs), block phases addressing the flush's block table at their
 // op_begin.
+// `prefetch`: software-pipelined tiles -- a thread's share of the next tile
+// is loaded into registers while the phases of the current one run.
 std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, bool c64, int M,
-                             int R, int SW);
+                             int R, int SW, bool prefetch = false);
 
 // Compile a source with hipRTC for gfx950 (no device needed); returns the
 // code object, or throws with the compiler log.

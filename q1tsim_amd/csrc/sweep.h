@@ -16,8 +16,10 @@ struct PlanStep {
     std::vector<size_t> gates;      // indices into the queue, in application order
 };
 
-// Default tile geometry per precision.
+// Default tile geometry per precision (interpreter kernels), and with the
+// kernels specialised to each sweep program.
 TileGeom tile_geometry(Precision p);
+TileGeom tile_geometry_specialised(Precision p);
 // Is (M, R) one of the compiled tile shapes for this precision?
 bool tile_shape_supported(Precision p, int M, int R);
 
@@ -81,7 +83,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
 // `q` would run (plain programs; with fusion_k > 0 greedy fixed-width
 // fusion), block offsets as in the flush's merged table.  Host only.
 std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int n, Precision prec, int fusion_k,
-                                          bool search);
+                                          bool search, bool prefetch = false);
 
 // Apply the pending gates to columns [col_begin, col_end) of the state with
 // the planned schedule: runs of gates whose qubits fit one tile are applied

@@ -322,9 +322,9 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
 }
 
 std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int n, Precision prec, int fusion_k,
-                                          bool search)
+                                          bool search, bool prefetch)
 {
-    const TileGeom g = tile_geometry(prec);
+    const TileGeom g = tile_geometry_specialised(prec);
     const int K = prec == PREC_C64 && fusion_k > 0 ? std::min(6, std::max(3, fusion_k)) : 0;
     const std::vector<PlanStep> steps = plan_gates(q, n, g, K, search);
     std::vector<std::string> out;
@@ -335,7 +335,7 @@ std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int
         for (SwPhase& ph : p.phases)
             if (ph.kind == 1) ph.op_begin += uint32_t(nblk_floats);
         nblk_floats += p.blocks.size();
-        out.push_back(jit_sweep_source(p.phases, p.ops, prec == PREC_C64, g.M, g.R, g.M == 13 ? 5 : 4));
+        out.push_back(jit_sweep_source(p.phases, p.ops, prec == PREC_C64, g.M, g.R, g.M == 13 ? 5 : 4, prefetch));
     }
     return out;
 }
@@ -391,7 +391,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         for (size_t s = 0; s < steps.size(); ++s)
             if (steps[s].sweep) {
                 src[s] = jit_sweep_source(progs[s].phases, progs[s].ops, st.precision() == PREC_C64, g.M, g.R,
-                                          g.M == 13 ? 5 : 4);
+                                          g.M == 13 ? 5 : 4, st.jit_prefetch());
                 all.push_back(src[s]);
             }
         JitCache& jc = JitCache::get();
@@ -439,6 +439,9 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             stats.gates += step.gates.size();
             stats.bytes += 2.0 * S;
             stats.phases += progs[s].phases.size();
+            for (const SwPhase& ph : progs[s].phases)
+                if (ph.kind == 1)
+                    stats.mfma_flops += std::ldexp(double(col_end - col_begin), n + int(ph.reg_mask) + 3);
         }
         stats.blocks += uint64_t(nblocks);
     };
@@ -508,6 +511,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         stats.gates += d.gates;
         stats.blocks += d.blocks;
         stats.phases += d.phases;
+        stats.mfma_flops += d.mfma_flops;
         st.note_graph(false);
         return;
     }
@@ -538,6 +542,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
     fg.delta.sweeps = stats.sweeps - before.sweeps;
     fg.delta.gates = stats.gates - before.gates;
     fg.delta.blocks = stats.blocks - before.blocks;
+    fg.delta.mfma_flops = stats.mfma_flops - before.mfma_flops;
     fg.delta.phases = stats.phases - before.phases;
     st.note_graph(true);
     SW_HIP_OK(hipGraphLaunch(exec, st.stream()));

@@ -456,6 +456,23 @@ template <int M, int R> __device__ __forceinline__ void tile_addr(const SweepArg
     }
 }
 
+// The product, in program order, of the factors of the tile-factor ops
+// [begin, end) for the tile at `base`: a phase-on-ones op contributes its
+// phase when all its (outer) bits are set, a one-target diagonal on an outer
+// bit d0 or d1 by that bit, each only when its outer controls are set.
+template <typename V>
+__device__ __forceinline__ V tile_factor(const SwOp* ops, uint32_t begin, uint32_t end, uint64_t base)
+{
+    V f = mkc<V>(1, 0);
+    for (uint32_t oi = begin; oi < end; ++oi) {
+        const SwOp op = load_uniform(ops, oi);
+        if ((base & op.co) != op.co) continue;
+        if (op.type == SW_PHASE) f = cmul(f, opm<V>(op, 0));
+        else f = cmul(f, sel(((base >> op.dglobal) & 1) != 0, opm<V>(op, 1), opm<V>(op, 0)));
+    }
+    return f;
+}
+
 // The program's phases on a tile already in LDS, then LDS -> HBM.  WIDE
 // kernels also hold the 5- and 6-qubit dense blocks; the others only 3 and 4,
 // which keeps the wide blocks' registers out of the common kernels.
@@ -466,11 +483,18 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
     constexpr int NT = 1 << (M - R), NR = 1 << R;
     using Rl = typename RealOf<V>::type;
     const int tid = threadIdx.x;
+    V fac = mkc<V>(1, 0);
+    bool scaled = false;
     {
         for (int p = 0; p < sa.nphases; ++p) {
             // the program is read through the constant address space: scalar
             // loads, whatever else the kernel writes (including the DMA)
             const SwPhase ph = load_uniform(sa.phases, uint32_t(p));
+            if (ph.kind == 2) {
+                fac = tile_factor<V>(sa.ops, ph.op_begin, ph.op_end, base);
+                scaled = true;
+                continue;
+            }
             if constexpr (sizeof(V) == 8) {
                 if (ph.kind == 1) {
                     // fused dense block: K = reg_mask, matrix at blocks + op_begin
@@ -560,8 +584,13 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
             __syncthreads();
         }
 
+        if (scaled) {
 #pragma unroll
-        for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
+            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = cmul(fac, lds[swz<SW, M>(tid + k * NT)]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
+        }
         __syncthreads();
     }
 }

@@ -46,7 +46,11 @@ struct SwOp {
 struct SwPhase {
     uint32_t op_begin, op_end;
     uint32_t reg_mask;   // local tile bits held in registers during this phase
-    uint32_t kind;       // 0: register phase, 1: fused dense block on LDS
+    uint32_t kind;       // 0: register phase, 1: fused dense block on LDS,
+                         // 2: tile factors -- ops [op_begin, op_end) are
+                         // diagonal ops on bits outside the tile only; the
+                         // tile is scaled by the product of their factors
+                         // as it is stored (the program's last phase)
     uint8_t rpos[8];     // local positions of the register bits (ascending)
     uint8_t tbit[16];    // local positions of the thread bits (ascending)
     // register phases: an affine map over GF(2) of the local index, applied

@@ -142,10 +142,11 @@ class GpuVectorState:
         uploaded before each replay)."""
         check(lib().q1t_gvs_set_graph(self.h, int(on)))
 
-    def set_jit(self, on=True):
+    def set_jit(self, on=True, prefetch=False):
         """Sweep kernels specialised to each sweep program (compiled with
-        hipRTC on first use, cached per process by program structure)."""
-        check(lib().q1t_gvs_set_jit(self.h, 1 if on else 0))
+        hipRTC on first use, cached per process by program structure);
+        `prefetch`: the next tile is loaded while the current one runs."""
+        check(lib().q1t_gvs_set_jit(self.h, (2 if prefetch else 1) if on else 0))
 
     def graph_stats(self):
         """(graphs captured, replays launched) over the state's life."""
@@ -237,7 +238,8 @@ class GpuVectorState:
         s = _lib.Stats()
         check(lib().q1t_gvs_last_stats(self.h, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates, "blocks": s.blocks, "phases": s.phases}
+                "gates": s.gates, "blocks": s.blocks, "phases": s.phases,
+                "mfma_flops": s.mfma_flops}
 
     def set_profiling(self, on=True):
         check(lib().q1t_gvs_set_profiling(self.h, 1 if on else 0))

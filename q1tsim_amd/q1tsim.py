@@ -267,16 +267,16 @@ class Circuit(object):
         previous execute's, with reference parameters re-read."""
         _lib.check(self.__sim.q1t_circuit_set_graph(self.__ptr, 1 if on else 0))
 
-    def set_jit(self, on=True):
+    def set_jit(self, on=True, prefetch=False):
         """Sweep kernels specialised to the state's sweep programs."""
-        _lib.check(self.__sim.q1t_circuit_set_jit(self.__ptr, 1 if on else 0))
+        _lib.check(self.__sim.q1t_circuit_set_jit(self.__ptr, (2 if prefetch else 1) if on else 0))
 
-    def jit_compile(self, precision="c128", fusion_k=0, out_dir=None):
+    def jit_compile(self, precision="c128", fusion_k=0, out_dir=None, prefetch=False):
         """Plan this gate-only circuit on the host and compile the kernels
         specialised to its sweeps (no GPU needed): (kernels, seconds)."""
         prec = {"c128": 0, "c64": 1}[precision]
         n, sec = ctypes.c_size_t(0), ctypes.c_double(0.0)
-        _lib.check(self.__sim.q1t_circuit_jit_compile(self.__ptr, prec, int(fusion_k),
+        _lib.check(self.__sim.q1t_circuit_jit_compile(self.__ptr, prec, int(fusion_k), 2 if prefetch else 1,
                                                       out_dir.encode() if out_dir else None,
                                                       ctypes.byref(n), ctypes.byref(sec)))
         return int(n.value), float(sec.value)
@@ -323,7 +323,8 @@ class Circuit(object):
         s = _lib.Stats()
         _lib.check(self.__sim.q1t_circuit_last_stats(self.__ptr, ctypes.byref(s)))
         return {"gate_seconds": s.gate_seconds, "bytes": s.bytes, "launches": s.launches, "sweeps": s.sweeps,
-                "gates": s.gates, "blocks": s.blocks, "phases": s.phases}
+                "gates": s.gates, "blocks": s.blocks, "phases": s.phases,
+                "mfma_flops": s.mfma_flops}
 
     def histogram_vec(self):
         n = 1 << self.nr_cbits()

@@ -24,14 +24,15 @@ def mixed_circuit(n, seed):
     return ops
 
 
-@pytest.mark.parametrize("precision,fusion_k", [("c128", 0), ("c64", 0), ("c64", 4), ("c64", 6)])
-def test_specialised_kernels_compile(precision, fusion_k):
+@pytest.mark.parametrize("precision,fusion_k,prefetch", [("c128", 0, False), ("c64", 0, False), ("c64", 4, False),
+                                                         ("c64", 6, False), ("c64", 0, True)])
+def test_specialised_kernels_compile(precision, fusion_k, prefetch):
     n = 16
     c = Circuit(n, 0)
     for name, bits, params in mixed_circuit(n, 3):
         c.add_gate(name, bits, params if params else None)
     with tempfile.TemporaryDirectory() as d:
-        nk, secs = c.jit_compile(precision, fusion_k, d)
+        nk, secs = c.jit_compile(precision, fusion_k, d, prefetch=prefetch)
         assert nk >= 1
         for i in range(nk):
             co = os.path.join(d, "sweep_%d.co" % i)
@@ -41,9 +42,10 @@ def test_specialised_kernels_compile(precision, fusion_k):
             notes = subprocess.run([READELF, "--notes", co], capture_output=True, text=True).stdout
             m = re.search(r"\.private_segment_fixed_size:\s+(\d+)", notes)
             assert m and int(m.group(1)) == 0, "scratch memory in %s" % co
+            src = open(os.path.join(d, "sweep_%d.hip" % i)).read()
             if fusion_k:
-                src = open(os.path.join(d, "sweep_%d.hip" % i)).read()
                 assert "block_mfma" in src or "dense_pairs" in src or "phase_op" in src
+            assert ("pf0" in src) == prefetch
 
 
 def test_jit_compile_rejects_non_gate_ops():
