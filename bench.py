@@ -285,8 +285,9 @@ def main():
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batch leg (both precisions), one JSON line; for kernel-trace profiles")
     ap.add_argument("--profile-workload", action="store_true",
-                    help="one pass for rocprofv3 counter runs: k_scale over the c128 state (a read+write "
-                         "of known size, for scaling FETCH_SIZE/WRITE_SIZE), then the plain circuit once")
+                    help="for rocprofv3 counter runs: k_scale over the c128 state (a read+write of known size, "
+                         "for scaling FETCH_SIZE/WRITE_SIZE), then the plain circuit twice (specialised kernels: "
+                         "the first pass compiles them, the second is the profiled one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU sharding is not part of this build (run with --gpus 1)",
@@ -304,8 +305,11 @@ def main():
         n = args.n
         st = GpuVectorState(n, 1, "c128")
         st.set_plan(0, True)
+        st.set_jit(True)
         st.set_random_normal(3)   # k_random_normal, k_norm_partial, k_scale (reads and writes 16 * 2^n bytes)
-        apply_circuit(st, workloads.random_layered(n, args.depth, 1))
+        lay = workloads.random_layered(n, args.depth, 1)
+        apply_circuit(st, lay)    # compiles the specialised sweep kernels
+        apply_circuit(st, lay)    # the profiled pass: the last `sweeps` q1t_jit_sweep_c128 dispatches
         s = st.stats()
         print(json.dumps({"profile_workload": True, "n": n, "state_bytes": 16 << n, "sweeps": s["sweeps"],
                           "sweep_bytes_per_launch": 2 * (16 << n)}))
@@ -345,7 +349,6 @@ def main():
     interp = GpuVectorState(n, 1, "c128")
     interp.set_plan(0, True)
     t_int, _, _, prof_int = timed_steps(interp, ops, 2, 1)
-    infid_ji, maxabs_ji = compare(plain, interp)
     del interp
 
     # complex64: the per-gate path, greedy fixed-width fusion into k-qubit
@@ -437,8 +440,7 @@ def main():
                        "kernels": "specialised per sweep program (hipRTC)", "jit": jit_info,
                        "interpreter_ms": round(1e3 * float(np.median(t_int)), 3),
                        "interpreter_kernels": {k: {"launches": v[0], "ms": round(v[1], 3)}
-                                               for k, v in prof_int.items()},
-                       "interpreter_vs_specialised": {"infidelity": infid_ji, "max_abs": maxabs_ji}},
+                                               for k, v in prof_int.items()}},
         "fused_c64": {"ms": ms_fused, "planner": "learned (cost model trained in this run)",
                       "sweeps": c64["learned"]["sweeps"], "blocks": c64["learned"]["blocks"],
                       "GBps": round(c64["learned"]["hbm_bytes"] / (ms_fused * 1e-3) / 1e9, 1),

@@ -115,6 +115,12 @@ int q1t_gvs_marginal_w0_noalloc(q1t_gvs* s, size_t qbit, double* w0s, size_t len
 int q1t_gvs_measure_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len, const q1t_rng* rng);
 /* QuState::peek_into (qustate.rs:62, vectorstate.rs:346-393). */
 int q1t_gvs_peek_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len, const q1t_rng* rng);
+/* Build-added: measure (collapse = 1) or peek one qubit in the 'Z', 'X' or
+ * 'Y' basis, the op loop's basis change (circuit.rs:669-718: H / Sdg.H
+ * before, H / H.S after) folded into the marginal and collapse kernels: one
+ * read pass for the marginal, one pass for the collapse, no gate passes. */
+int q1t_gvs_measure_basis_into(q1t_gvs* s, size_t qbit, size_t cbit, char basis, uint64_t* res, size_t res_len,
+                               const q1t_rng* rng, int collapse);
 /* QuState::measure_all_into (qustate.rs:50, vectorstate.rs:340-344). */
 int q1t_gvs_measure_all_into(q1t_gvs* s, const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,
                              const q1t_rng* rng);

@@ -135,6 +135,7 @@ SIGNATURES = {
     "q1t_gvs_measure_with_n0": (_i32, [_vp, _sz, _sz, _ps, _pd, _sz, _pu64, _sz, _i32]),
     "q1t_gvs_measure_into": (_i32, [_vp, _sz, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),
     "q1t_gvs_peek_into": (_i32, [_vp, _sz, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),
+    "q1t_gvs_measure_basis_into": (_i32, [_vp, _sz, _sz, ctypes.c_char, _pu64, _sz, ctypes.POINTER(Rng), _i32]),
     "q1t_gvs_measure_all_into": (_i32, [_vp, _ps, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),
     "q1t_gvs_peek_all_into": (_i32, [_vp, _ps, _sz, _pu64, _sz, ctypes.POINTER(Rng)]),
     "q1t_gvs_measure_all_uniforms": (_i32, [_vp, _ps, _sz, _pd, _pu64, _sz, _i32]),

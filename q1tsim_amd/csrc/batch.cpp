@@ -224,7 +224,7 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             for (size_t k = 0; k < active.size(); ++k) {
                 if (!more[k]) continue;
                 Circuit* c = active[k];
-                const char kd = c->next_is_z_measure() ? 1 : c->next_is_z_measure_all() ? 2 : 0;
+                const char kd = c->next_is_measure() ? 1 : c->next_is_z_measure_all() ? 2 : 0;
                 if (!kd) continue;
                 measured[k] = 1;
                 meas.push_back(c);

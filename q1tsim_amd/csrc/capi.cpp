@@ -695,6 +695,19 @@ extern "C" int q1t_gvs_measure_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64
     });
 }
 
+extern "C" int q1t_gvs_measure_basis_into(q1t_gvs* s, size_t qbit, size_t cbit, char basis, uint64_t* res,
+                                          size_t res_len, const q1t_rng* rng, int collapse)
+{
+    NEED(s);
+    NEED(res);
+    return guard([&] {
+        const int b = basis == 'Z' || basis == 'z' ? 0 : basis == 'X' || basis == 'x' ? 1 : basis == 'Y' || basis == 'y' ? 2 : -1;
+        if (b < 0) throw Error(ERR_INVALID_ARGUMENT, std::string("Unknown measurement basis '") + basis + "'");
+        RngSource r = rng_of(rng);
+        s->s.measure_into(qbit, cbit, res, res_len, r, collapse != 0, b);
+    });
+}
+
 extern "C" int q1t_gvs_peek_into(q1t_gvs* s, size_t qbit, size_t cbit, uint64_t* res, size_t res_len,
                                  const q1t_rng* rng)
 {

@@ -182,17 +182,19 @@ void Circuit::run_next_op()
     ++cursor_;
 }
 
-bool Circuit::next_is_z_measure() const
+int Circuit::basis_code(char basis) { return basis == 'X' ? 1 : basis == 'Y' ? 2 : 0; }
+
+bool Circuit::next_is_measure() const
 {
     if (cursor_ >= ops_.size()) return false;
     const CircuitOp& op = ops_[cursor_];
-    return (op.type == OpType::Measure || op.type == OpType::Peek) && op.basis == 'Z';
+    return op.type == OpType::Measure || op.type == OpType::Peek;
 }
 
 void Circuit::measure_begin(double* host)
 {
     const CircuitOp& op = ops_[cursor_];
-    state_->marginal_w0_async(op.qbit, host);
+    state_->marginal_w0_async(op.qbit, host, basis_code(op.basis));
 }
 
 // The second half of measure_into (vectorstate.rs:264-275): a Binomial draw
@@ -202,7 +204,7 @@ void Circuit::measure_end(const double* host)
     const CircuitOp& op = ops_[cursor_];
     GpuVectorState& st = *state_;
     if (c_state_.size() < st.nr_shots()) throw err_not_enough_space(c_state_.size(), st.nr_shots());
-    std::vector<double> w0 = st.marginal_w0_collect(op.qbit, host);
+    std::vector<double> w0 = st.marginal_w0_collect(op.qbit, host, basis_code(op.basis));
     RngSource rng = rng_.source();
     const std::vector<size_t>& counts = st.counts();
     std::vector<size_t> n0s(counts.size());
@@ -214,7 +216,7 @@ void Circuit::measure_end(const double* host)
         n0s[c] = n0;
     }
     st.measure_with_n0(op.qbit, op.cbit, n0s.data(), w0, c_state_.data(), c_state_.size(),
-                       op.type == OpType::Measure);
+                       op.type == OpType::Measure, basis_code(op.basis));
     ++cursor_;
 }
 
@@ -283,31 +285,28 @@ void Circuit::run_op(const CircuitOp& op)
     }
     case OpType::Measure:
     case OpType::Peek: {
+        // X: H, measure, H; Y: Sdg, H, measure, H, S (circuit.rs:669-718) --
+        // the basis change is folded into the marginal and collapse kernels
         const bool collapse = op.type == OpType::Measure;
-        if (op.basis == 'X') apply_named(st, "h", op.qbit);
-        if (op.basis == 'Y') {
-            apply_named(st, "sdg", op.qbit);
-            apply_named(st, "h", op.qbit);
-        }
-        st.measure_into(op.qbit, op.cbit, c_state_.data(), c_state_.size(), rng, collapse);
-        if (op.basis == 'X') apply_named(st, "h", op.qbit);
-        if (op.basis == 'Y') {
-            apply_named(st, "h", op.qbit);
-            apply_named(st, "s", op.qbit);
-        }
+        st.measure_into(op.qbit, op.cbit, c_state_.data(), c_state_.size(), rng, collapse, basis_code(op.basis));
         break;
     }
     case OpType::MeasureAll:
     case OpType::PeekAll: {
+        // X: H on all, measure_all, H on all; Y: Sdg, H on all, measure_all,
+        // H, S on all (circuit.rs:686-748).  After a collapse the columns are
+        // basis vectors, and the rotation back is kept with them in closed
+        // form (no passes); a peek rotates the state back.
         const bool collapse = op.type == OpType::MeasureAll;
         if (op.basis == 'X') apply_named_all(st, "h");
         if (op.basis == 'Y') {
             apply_named_all(st, "sdg");
             apply_named_all(st, "h");
         }
-        st.measure_all_into(op.cbits.data(), op.cbits.size(), c_state_.data(), c_state_.size(), rng, collapse);
-        if (op.basis == 'X') apply_named_all(st, "h");
-        if (op.basis == 'Y') {
+        st.measure_all_into(op.cbits.data(), op.cbits.size(), c_state_.data(), c_state_.size(), rng, collapse,
+                            collapse ? basis_code(op.basis) : 0);
+        if (!collapse && op.basis == 'X') apply_named_all(st, "h");
+        if (!collapse && op.basis == 'Y') {
             apply_named_all(st, "h");
             apply_named_all(st, "s");
         }

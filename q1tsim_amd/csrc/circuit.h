@@ -62,10 +62,12 @@ public:
     void run_next_op();
     void finish();
     void release_state() { state_.reset(); }
-    // The op at the cursor is a Z-basis measure or peek: it can run in two
-    // steps, the marginal of many circuits' states enqueued before one
-    // synchronisation (batches).  measure_end completes it and advances.
-    bool next_is_z_measure() const;
+    // The op at the cursor is a measure or peek of one qubit (any basis): it
+    // can run in two steps, the marginal of many circuits' states enqueued
+    // before one synchronisation (batches).  measure_end completes it and
+    // advances.
+    bool next_is_measure() const;
+    static int basis_code(char basis);   // 'Z' / 'X' / 'Y' -> 0 / 1 / 2
     size_t measure_partials() const { return state_->marginal_partials(); }
     void measure_begin(double* host);
     void measure_end(const double* host);

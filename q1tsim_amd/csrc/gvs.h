@@ -144,7 +144,9 @@ public:
     void apply_unary_gate_all(const cdouble* mat2, const std::string& desc);
     void apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
                                 size_t nr_affected, const size_t* qbits, size_t nbits, const std::string& desc);
-    std::vector<double> marginal_w0(size_t qbit);
+    // basis: 0 = Z, 1 = X, 2 = Y (the X/Y-basis change of circuit.rs:669-748
+    // folded into the marginal and collapse passes)
+    std::vector<double> marginal_w0(size_t qbit, int basis = 0);
     // marginal_w0 without any device allocation: partial sums in a fixed
     // file-scope device array (at most kNoAllocMaxCols columns), summed on
     // the device in the host's order (bit-identical results).  Needs no
@@ -154,15 +156,20 @@ public:
     // into `host` (pinned, >= marginal_partials() doubles); once the stream
     // has synchronised, collect gives exactly what marginal_w0 gives.
     size_t marginal_partials() const;
-    void marginal_w0_async(size_t qbit, double* host);
-    std::vector<double> marginal_w0_collect(size_t qbit, const double* host) const;
+    void marginal_w0_async(size_t qbit, double* host, int basis = 0);
+    std::vector<double> marginal_w0_collect(size_t qbit, const double* host, int basis = 0) const;
     void measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
-                         uint64_t* res, size_t res_len, bool collapse);
-    void measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng, bool collapse);
+                         uint64_t* res, size_t res_len, bool collapse, int basis = 0);
+    void measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng, bool collapse,
+                      int basis = 0);
+    // rot: the basis a collapsing measure_all was taken in (0 = Z, 1 = X,
+    // 2 = Y; the caller rotated the state before): the collapsed columns are
+    // kept as basis vectors rotated back by H^n (X) or S^n H^n (Y), written
+    // out in closed form only when dense amplitudes are needed
     void measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
-                              size_t res_len, bool collapse);
+                              size_t res_len, bool collapse, int rot = 0);
     void measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len, RngSource& rng,
-                          bool collapse);
+                          bool collapse, int rot = 0);
     // measure_all in two steps (batches): `host` (pinned, measure_all_host_bytes)
     // starts with the nr_shots uniforms; async enqueues the sampling of every
     // column and the copies back; once the stream has synchronised, finish
@@ -170,7 +177,7 @@ public:
     size_t measure_all_host_bytes() const;
     void measure_all_async(void* host);
     void measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res, size_t res_len,
-                            bool collapse);
+                            bool collapse, int rot = 0);
     void reset(size_t bit, RngSource& rng);
     void reset_all();
 
@@ -280,6 +287,7 @@ private:
     Precision prec_;
     std::vector<size_t> counts_;
     bool basis_ = false;                  // columns are basis vectors |basis_idx_[c]>
+    int basis_rot_ = 0;                   // ... rotated by H^n (1) or S^n H^n (2)
     std::vector<uint64_t> basis_idx_;
     void* amp_ = nullptr;                 // column-major: column c at amp_ + c * 2^n elements
     size_t amp_cols_ = 0;                 // allocated columns
@@ -332,8 +340,9 @@ private:
     uint64_t dim() const { return uint64_t(1) << nr_bits_; }
     void check_qbit(size_t q) const;
     void alloc_cols(size_t ncols);
-    void launch_marginal(size_t qbit, double* part);
-    std::vector<double> w0_from_partials(size_t qbit, const double* h) const;
+    void launch_marginal(size_t qbit, double* part, int basis = 0);
+    std::vector<double> w0_from_partials(size_t qbit, const double* h, int basis = 0) const;
+    static void check_basis(int basis);
     void* dev_alloc(size_t bytes);
     void dev_release(void* p, size_t bytes);
     void* small_buf(size_t bytes);

@@ -219,6 +219,109 @@ __global__ void __launch_bounds__(kBlock) k_norm_partial(const V* __restrict__ a
     if (threadIdx.x == 0) partial[uint64_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
 }
 
+// Measurement in the X or Y basis without the basis-change passes: the
+// reference measures H.psi (X) or H.Sdg.psi (Y) on the qubit and rotates back
+// (circuit.rs:669-748).  On a pair (a0, a1) of the qubit, the rotated
+// amplitude of outcome 0 is b0 = (a0 + a1)/sqrt2 (X) or (a0 - i a1)/sqrt2 (Y),
+// of outcome 1 b1 = (a0 - a1)/sqrt2 or (a0 + i a1)/sqrt2.
+enum MeasBasis : int { BASIS_Z = 0, BASIS_X = 1, BASIS_Y = 2 };
+
+// Partial sums of |b0|^2 over the pairs of bit p (FP64), per column.
+template <typename V, int B>
+__global__ void __launch_bounds__(kBlock) k_norm_partial_basis(const V* __restrict__ amp, uint64_t col_stride,
+                                                               uint64_t nwork, int p, double* __restrict__ partial)
+{
+    const V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t bit = uint64_t(1) << p;
+    double acc = 0.0;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1));
+        const V x0 = a[i0], x1 = a[i0 | bit];
+        double re, im;
+        if constexpr (B == BASIS_X) {
+            re = double(x0.x) + double(x1.x);
+            im = double(x0.y) + double(x1.y);
+        } else {   // a0 - i a1
+            re = double(x0.x) + double(x1.y);
+            im = double(x0.y) - double(x1.x);
+        }
+        acc += 0.5 * __fma_rn(re, re, im * im);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[uint64_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// Collapse onto outcome 0 (dst0, scale s0 = 1/sqrt(w0)) and/or 1 (dst1,
+// s1 = 1/sqrt(1 - w0)) of an X- or Y-basis measurement, the basis change back
+// included: X, outcome 0: a0' = a1' = (a0 + a1) s0/2; outcome 1:
+// a0' = -a1' = (a0 - a1) s1/2.  Y, outcome 0: u = a0 - i a1, a0' = u s0/2,
+// a1' = i u s0/2; outcome 1: v = a0 + i a1, a0' = v s1/2, a1' = -i v s1/2.
+template <typename V, int B>
+__global__ void __launch_bounds__(kBlock) k_collapse_basis(const V* src, V* dst0, V* dst1, uint64_t nwork, int p,
+                                                           typename RealOf<V>::type s0,
+                                                           typename RealOf<V>::type s1)
+{
+    using Rl = typename RealOf<V>::type;
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t bit = uint64_t(1) << p;
+    const Rl h0 = Rl(0.5) * s0, h1 = Rl(0.5) * s1;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
+        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1)), i1 = i0 | bit;
+        const V x0 = src[i0], x1 = src[i1];
+        if constexpr (B == BASIS_X) {
+            const V u = mkc<V>((x0.x + x1.x) * h0, (x0.y + x1.y) * h0);
+            const V v = mkc<V>((x0.x - x1.x) * h1, (x0.y - x1.y) * h1);
+            if (dst0) {
+                dst0[i0] = u;
+                dst0[i1] = u;
+            }
+            if (dst1) {
+                dst1[i0] = v;
+                dst1[i1] = mkc<V>(-v.x, -v.y);
+            }
+        } else {
+            const V u = mkc<V>((x0.x + x1.y) * h0, (x0.y - x1.x) * h0);   // (a0 - i a1) s0/2
+            const V v = mkc<V>((x0.x - x1.y) * h1, (x0.y + x1.x) * h1);   // (a0 + i a1) s1/2
+            if (dst0) {
+                dst0[i0] = u;
+                dst0[i1] = mkc<V>(-u.y, u.x);   // i u
+            }
+            if (dst1) {
+                dst1[i0] = v;
+                dst1[i1] = mkc<V>(v.y, -v.x);   // -i v
+            }
+        }
+    }
+}
+
+// Dense columns of compact basis columns: column y holds |k> with k =
+// ks[y] (rot 0), H^n|k> (rot 1, after an X-basis measure_all) or
+// S^n H^n|k> (rot 2, Y basis): amplitude j = (-1)^popcount(j&k) i^popcount(j)
+// (the i factor for rot 2 only) times `mag` = 2^(-n/2); rot 0 writes the one
+// amplitude 1 and zeros.
+template <typename V>
+__global__ void __launch_bounds__(kBlock) k_basis_columns(V* __restrict__ amp, uint64_t col_stride, uint64_t dim,
+                                                          const uint64_t* __restrict__ ks, int rot,
+                                                          typename RealOf<V>::type mag)
+{
+    using Rl = typename RealOf<V>::type;
+    V* a = amp + uint64_t(blockIdx.y) * col_stride;
+    const uint64_t k = ks[blockIdx.y];
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < dim; j += step) {
+        V v;
+        if (rot == 0) {
+            v = mkc<V>(j == k ? Rl(1) : Rl(0), Rl(0));
+        } else {
+            const Rl s = (__popcll(j & k) & 1) ? -mag : mag;
+            const int q = rot == 2 ? (__popcll(j) & 3) : 0;   // i^q
+            v = q == 0 ? mkc<V>(s, 0) : q == 1 ? mkc<V>(0, s) : q == 2 ? mkc<V>(-s, 0) : mkc<V>(0, -s);
+        }
+        a[j] = v;
+    }
+}
+
 // Level sums of the canonical sampling tree: out[g] = left fold of
 // in[16g .. 16g+15].  A workgroup folds 4096 inputs; loads are coalesced
 // into LDS in a transposed, padded layout so each thread reads its 16
@@ -810,19 +913,26 @@ void GpuVectorState::materialize()
         throw Error(ERR_OUT_OF_MEMORY, "Not enough device memory to expand " + std::to_string(nc) +
                                            " measured basis columns into dense columns");
     alloc_cols(nc);
-    HIP_OK(hipMemsetAsync(amp_, 0, nc * dim() * elem_bytes(), stream_));
-    for (size_t c = 0; c < nc; ++c) {
-        char* p = static_cast<char*>(col_ptr(c)) + basis_idx_[c] * elem_bytes();
-        if (prec_ == PREC_C128) {
-            static const double one[2] = {1.0, 0.0};
-            HIP_OK(hipMemcpyAsync(p, one, sizeof(one), hipMemcpyHostToDevice, stream_));
-        } else {
-            static const float one[2] = {1.0f, 0.0f};
-            HIP_OK(hipMemcpyAsync(p, one, sizeof(one), hipMemcpyHostToDevice, stream_));
-        }
+    // the basis indices, then one launch writes every column (kernel writes
+    // only: no memset node if a caller captures it)
+    uint64_t* ks = static_cast<uint64_t*>(small_buf(nc * sizeof(uint64_t)));
+    HIP_OK(hipMemcpyAsync(ks, basis_idx_.data(), nc * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+    const double mag = (nr_bits_ & 1) ? std::ldexp(0.70710678118654752440, -int(nr_bits_ / 2))
+                                      : std::ldexp(1.0, -int(nr_bits_ / 2));
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
+        const dim3 g(grid_for(dim()), ny);
+        if (prec_ == PREC_C128)
+            hipLaunchKernelGGL(k_basis_columns<double2>, g, dim3(kBlock), 0, stream_,
+                               static_cast<double2*>(col_ptr(c0)), dim(), dim(), ks + c0, basis_rot_, mag);
+        else
+            hipLaunchKernelGGL(k_basis_columns<float2>, g, dim3(kBlock), 0, stream_,
+                               static_cast<float2*>(col_ptr(c0)), dim(), dim(), ks + c0, basis_rot_, float(mag));
+        HIP_OK(hipGetLastError());
     }
     HIP_OK(hipStreamSynchronize(stream_));
     basis_ = false;
+    basis_rot_ = 0;
     basis_idx_.clear();
 }
 
@@ -1072,7 +1182,7 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
 // column, into `part` (device) on the state's stream.
 static constexpr unsigned kNormBlocks = 256;
 
-void GpuVectorState::launch_marginal(size_t qbit, double* part)
+void GpuVectorState::launch_marginal(size_t qbit, double* part, int basis)
 {
     const size_t nc = counts_.size();
     const int p = int(nr_bits_ - 1 - qbit);
@@ -1080,6 +1190,28 @@ void GpuVectorState::launch_marginal(size_t qbit, double* part)
     const uint64_t nwork = dim() / 2;
     for (size_t c0 = 0; c0 < nc; c0 += 65535) {
         const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
+        const dim3 grid(kNormBlocks, ny);
+        if (basis != BASIS_Z) {
+            if (prec_ == PREC_C128) {
+                const double2* a = static_cast<const double2*>(col_ptr(c0));
+                if (basis == BASIS_X)
+                    hipLaunchKernelGGL((k_norm_partial_basis<double2, BASIS_X>), grid, dim3(kBlock), 0, stream_, a,
+                                       dim(), nwork, p, part + c0 * kNormBlocks);
+                else
+                    hipLaunchKernelGGL((k_norm_partial_basis<double2, BASIS_Y>), grid, dim3(kBlock), 0, stream_, a,
+                                       dim(), nwork, p, part + c0 * kNormBlocks);
+            } else {
+                const float2* a = static_cast<const float2*>(col_ptr(c0));
+                if (basis == BASIS_X)
+                    hipLaunchKernelGGL((k_norm_partial_basis<float2, BASIS_X>), grid, dim3(kBlock), 0, stream_, a,
+                                       dim(), nwork, p, part + c0 * kNormBlocks);
+                else
+                    hipLaunchKernelGGL((k_norm_partial_basis<float2, BASIS_Y>), grid, dim3(kBlock), 0, stream_, a,
+                                       dim(), nwork, p, part + c0 * kNormBlocks);
+            }
+            HIP_OK(hipGetLastError());
+            continue;
+        }
         if (prec_ == PREC_C128)
             hipLaunchKernelGGL(k_norm_partial<double2>, dim3(kNormBlocks, ny), dim3(kBlock), 0, stream_,
                                static_cast<const double2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * kNormBlocks);
@@ -1091,13 +1223,17 @@ void GpuVectorState::launch_marginal(size_t qbit, double* part)
 }
 
 // w0 per column from the partials, summed in column-major block order.
-std::vector<double> GpuVectorState::w0_from_partials(size_t qbit, const double* h) const
+std::vector<double> GpuVectorState::w0_from_partials(size_t qbit, const double* h, int basis) const
 {
     const size_t nc = counts_.size();
     std::vector<double> w0(nc, 0.0);
     const int p = int(nr_bits_ - 1 - qbit);
     if (basis_) {
-        for (size_t c = 0; c < nc; ++c) w0[c] = ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+        // measured in the basis the columns were collapsed in, a basis
+        // vector gives its own bit; in another basis either outcome with
+        // probability 1/2
+        for (size_t c = 0; c < nc; ++c)
+            w0[c] = basis != basis_rot_ ? 0.5 : ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
         return w0;
     }
     for (size_t c = 0; c < nc; ++c)
@@ -1105,35 +1241,43 @@ std::vector<double> GpuVectorState::w0_from_partials(size_t qbit, const double* 
     return w0;
 }
 
-std::vector<double> GpuVectorState::marginal_w0(size_t qbit)
+std::vector<double> GpuVectorState::marginal_w0(size_t qbit, int basis)
 {
     check_qbit(qbit);
+    check_basis(basis);
     flush();
-    if (basis_) return w0_from_partials(qbit, nullptr);
+    if (basis_) return w0_from_partials(qbit, nullptr, basis);
     const size_t nc = counts_.size();
     double* part = partial_buf(size_t(kNormBlocks) * nc);
-    launch_marginal(qbit, part);
+    launch_marginal(qbit, part, basis);
     std::vector<double> h(size_t(kNormBlocks) * nc);
     HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    return w0_from_partials(qbit, h.data());
+    return w0_from_partials(qbit, h.data(), basis);
+}
+
+void GpuVectorState::check_basis(int basis)
+{
+    if (basis != BASIS_Z && basis != BASIS_X && basis != BASIS_Y)
+        throw Error(ERR_INVALID_ARGUMENT, "Unknown measurement basis " + std::to_string(basis));
 }
 
 size_t GpuVectorState::marginal_partials() const { return size_t(kNormBlocks) * counts_.size(); }
 
-void GpuVectorState::marginal_w0_async(size_t qbit, double* host)
+void GpuVectorState::marginal_w0_async(size_t qbit, double* host, int basis)
 {
     check_qbit(qbit);
+    check_basis(basis);
     flush();
     if (basis_) return;
     double* part = partial_buf(marginal_partials());
-    launch_marginal(qbit, part);
+    launch_marginal(qbit, part, basis);
     HIP_OK(hipMemcpyAsync(host, part, marginal_partials() * sizeof(double), hipMemcpyDeviceToHost, stream_));
 }
 
-std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const double* host) const
+std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const double* host, int basis) const
 {
-    return w0_from_partials(qbit, host);
+    return w0_from_partials(qbit, host, basis);
 }
 
 // ------------------------------------------- allocation-free marginal w0
@@ -1191,7 +1335,7 @@ void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
         throw Error(ERR_INVALID_ARGUMENT, "Gates are pending on the state: flush it before the allocation-free marginal");
     if (basis_) {
         const int p = int(nr_bits_ - 1 - qbit);
-        for (size_t c = 0; c < nc; ++c) w0s[c] = ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+        for (size_t c = 0; c < nc; ++c) w0s[c] = basis_rot_ != 0 ? 0.5 : ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
         return;
     }
     if (nc > kNoAllocMaxCols)
@@ -1214,9 +1358,10 @@ void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
 }
 
 void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
-                                     uint64_t* res, size_t res_len, bool collapse)
+                                     uint64_t* res, size_t res_len, bool collapse, int basis)
 {
     check_qbit(qbit);
+    check_basis(basis);
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     const size_t nc = counts_.size();
     const uint64_t one_mask = uint64_t(1) << cbit, zero_mask = ~one_mask;
@@ -1230,10 +1375,34 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
     }
     if (!collapse) return;
     const int p = int(nr_bits_ - 1 - qbit);
-    if (basis_ && new_nr == nc) return;   // a basis column collapses onto itself
+    if (basis_ && new_nr == nc && basis == basis_rot_) return;   // a basis column collapses onto itself
     materialize();
     const uint64_t nwork = dim() / 2;
     auto launch = [&](const void* src, void* d0, void* d1, double s0, double s1) {
+        if (basis != BASIS_Z) {
+            const dim3 g(grid_for(nwork));
+            if (prec_ == PREC_C128) {
+                const double2* a = static_cast<const double2*>(src);
+                double2 *o0 = static_cast<double2*>(d0), *o1 = static_cast<double2*>(d1);
+                if (basis == BASIS_X)
+                    hipLaunchKernelGGL((k_collapse_basis<double2, BASIS_X>), g, dim3(kBlock), 0, stream_, a, o0, o1,
+                                       nwork, p, s0, s1);
+                else
+                    hipLaunchKernelGGL((k_collapse_basis<double2, BASIS_Y>), g, dim3(kBlock), 0, stream_, a, o0, o1,
+                                       nwork, p, s0, s1);
+            } else {
+                const float2* a = static_cast<const float2*>(src);
+                float2 *o0 = static_cast<float2*>(d0), *o1 = static_cast<float2*>(d1);
+                if (basis == BASIS_X)
+                    hipLaunchKernelGGL((k_collapse_basis<float2, BASIS_X>), g, dim3(kBlock), 0, stream_, a, o0, o1,
+                                       nwork, p, float(s0), float(s1));
+                else
+                    hipLaunchKernelGGL((k_collapse_basis<float2, BASIS_Y>), g, dim3(kBlock), 0, stream_, a, o0, o1,
+                                       nwork, p, float(s0), float(s1));
+            }
+            HIP_OK(hipGetLastError());
+            return;
+        }
         if (prec_ == PREC_C128)
             hipLaunchKernelGGL(k_collapse<double2>, dim3(grid_for(nwork)), dim3(kBlock), 0, stream_,
                                static_cast<const double2*>(src), static_cast<double2*>(d0), static_cast<double2*>(d1),
@@ -1285,11 +1454,11 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
 }
 
 void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng,
-                                  bool collapse)
+                                  bool collapse, int basis)
 {
     check_qbit(qbit);
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
-    std::vector<double> w0 = marginal_w0(qbit);
+    std::vector<double> w0 = marginal_w0(qbit, basis);
     // n0 ~ Binomial(count, min(w0, 1)) drawn as a count of Bernoulli trials,
     // one uniform per run, columns in order (vectorstate.rs:264-275).
     std::vector<size_t> n0s(counts_.size());
@@ -1300,7 +1469,7 @@ void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_
             if (rng.uniform() < p) ++n0;
         n0s[c] = n0;
     }
-    measure_with_n0(qbit, cbit, n0s.data(), w0, res, res_len, collapse);
+    measure_with_n0(qbit, cbit, n0s.data(), w0, res, res_len, collapse, basis);
 }
 
 // Levels of the canonical tree of one column: sizes shrink by 16 until <= 16.
@@ -1333,6 +1502,7 @@ size_t GpuVectorState::measure_all_host_bytes() const
 void GpuVectorState::measure_all_async(void* host)
 {
     flush();
+    if (basis_ && basis_rot_ != 0) materialize();   // rotated basis columns are dense in Z
     if (basis_) return;
     const size_t nc = counts_.size();
     double* hu = static_cast<double*>(host);
@@ -1422,8 +1592,9 @@ void GpuVectorState::measure_all_async(void* host)
 // on the indices measure_all_async sampled; outcomes grouped per column in
 // ascending index order.
 void GpuVectorState::measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res,
-                                        size_t res_len, bool collapse)
+                                        size_t res_len, bool collapse, int rot)
 {
+    check_basis(rot);
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
     const size_t nc = counts_.size();
@@ -1486,13 +1657,14 @@ void GpuVectorState::measure_all_finish(const size_t* cbits, size_t ncbits, cons
     }
     if (collapse) {
         basis_ = true;
+        basis_rot_ = rot;
         basis_idx_ = uidx;
         counts_.assign(ucnt.begin(), ucnt.end());
     }
 }
 
 void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
-                                          size_t res_len, bool collapse)
+                                          size_t res_len, bool collapse, int rot)
 {
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
@@ -1500,17 +1672,17 @@ void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, co
     std::copy(u01, u01 + nr_shots_, host.data());
     measure_all_async(host.data());
     HIP_OK(hipStreamSynchronize(stream_));
-    measure_all_finish(cbits, ncbits, host.data(), res, res_len, collapse);
+    measure_all_finish(cbits, ncbits, host.data(), res, res_len, collapse, rot);
 }
 
 void GpuVectorState::measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,
-                                      RngSource& rng, bool collapse)
+                                      RngSource& rng, bool collapse, int rot)
 {
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
     std::vector<double> u(nr_shots_);
     for (size_t r = 0; r < nr_shots_; ++r) u[r] = rng.uniform();
-    measure_all_uniforms(cbits, ncbits, u.data(), res, res_len, collapse);
+    measure_all_uniforms(cbits, ncbits, u.data(), res, res_len, collapse, rot);
 }
 
 void GpuVectorState::reset(size_t bit, RngSource& rng)
@@ -1530,11 +1702,12 @@ void GpuVectorState::read_amplitudes(size_t col, cdouble* out)
 {
     flush();
     if (col >= counts_.size()) throw Error(ERR_INVALID_ARGUMENT, "Column index out of range");
-    if (basis_) {
+    if (basis_ && basis_rot_ == 0) {
         std::fill(out, out + dim(), cdouble(0, 0));
         out[basis_idx_[col]] = cdouble(1, 0);
         return;
     }
+    materialize();
     if (prec_ == PREC_C128) {
         HIP_OK(hipMemcpyAsync(out, col_ptr(col), dim() * sizeof(cdouble), hipMemcpyDeviceToHost, stream_));
         HIP_OK(hipStreamSynchronize(stream_));

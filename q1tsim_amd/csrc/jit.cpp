@@ -137,6 +137,8 @@ static void emit_block_phase(std::ostringstream& o, const SwPhase& ph, int M)
     o << "        __syncthreads();\n    }\n";
 }
 
+const char* jit_kernel_name(bool c64) { return c64 ? "q1t_jit_sweep_c64" : "q1t_jit_sweep_c128"; }
+
 std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, bool c64, int M,
                              int R, int SW, bool prefetch)
 {
@@ -147,7 +149,7 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
     const int waves = M == 12 ? (R == 3 ? 4 : 2) : 2;
     o << "#include \"sweep_dev.h\"\nusing namespace q1t;\n";
     o << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") __attribute__((amdgpu_waves_per_eu(" << waves
-      << ")))\nq1t_jit_sweep(" << (c64 ? "float2" : "double2")
+      << ")))\n" << jit_kernel_name(c64) << "(" << (c64 ? "float2" : "double2")
       << "* __restrict__ amp, uint64_t col_stride, SweepArgs sa)\n{\n";
     o << "    typedef " << (c64 ? "float2" : "double2") << " V;\n";
     o << "    constexpr int M = " << M << ", R = " << R << ", SW = " << SW << ";\n";
@@ -387,7 +389,8 @@ hipFunction_t JitCache::function(const std::string& src, bool compile)
         throw Error(ERR_HIP, std::string("hipModuleLoadData of a sweep kernel failed: ") + hipGetErrorString(he));
     }
     hipFunction_t fn = nullptr;
-    he = hipModuleGetFunction(&fn, m, "q1t_jit_sweep");
+    const char* name = src.find(jit_kernel_name(true)) != std::string::npos ? jit_kernel_name(true) : jit_kernel_name(false);
+    he = hipModuleGetFunction(&fn, m, name);
     if (he != hipSuccess) {
         (void)hipGetLastError();
         throw Error(ERR_HIP, std::string("hipModuleGetFunction failed: ") + hipGetErrorString(he));

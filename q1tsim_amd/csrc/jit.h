@@ -21,7 +21,10 @@
 
 namespace q1t {
 
-// Source of the specialised kernel `q1t_jit_sweep` for one program: the same
+// Name of the specialised kernel: q1t_jit_sweep_c128 / q1t_jit_sweep_c64.
+const char* jit_kernel_name(bool c64);
+
+// Source of the specialised kernel (jit_kernel_name) for one program: the same
 // arguments as k_sweep<V, M, R, SW, *> (amplitudes, column stride,
 // SweepArgs), block phases addressing the flush's block table at their
 // op_begin.

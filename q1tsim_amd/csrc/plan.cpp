@@ -643,12 +643,14 @@ void compose(std::vector<cdouble>& U, int nb, const GateOp& op, const int* pos)
 }  // namespace
 
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
-                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search, std::vector<size_t>* order)
+                   std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search, std::vector<size_t>* order,
+                   bool factors)
 {
     const LocalMap lm(step.tile);
     std::vector<size_t> pending = step.gates;
     TileFactors tf;
-    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, order, &tf);
+    TileFactors* tfp = factors ? &tf : nullptr;
+    while (!pending.empty()) pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, order, tfp);
     append_factors(tf, M, phases, ops, order);
 }
 
@@ -669,11 +671,12 @@ std::vector<size_t> plan_order(const std::vector<PendingGate>& q, int n, const T
 
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
-                         int* nblocks, bool search)
+                         int* nblocks, bool search, bool factors)
 {
     const LocalMap lm(step.tile);
     std::vector<size_t> pending = step.gates;
     TileFactors tf;
+    TileFactors* tfp = factors ? &tf : nullptr;
     while (!pending.empty()) {
         // grow one block: gates whose bits are all in the tile, while the
         // union of their local bits stays within K (order kept per bit)
@@ -703,7 +706,7 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
             absorbed.push_back(i);
         }
         if (absorbed.empty()) {
-            pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, nullptr, &tf);
+            pending = one_reg_phase(q, pending, lm, M, R, phases, ops, search, nullptr, tfp);
             continue;
         }
         if (search && int(B.size()) == K) {

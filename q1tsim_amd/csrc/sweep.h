@@ -38,12 +38,16 @@ std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const
 // Register phases of one sweep (per-gate arithmetic).  `order`, if given,
 // receives the queue indices of the sweep's gates in the order the kernel
 // applies them (per phase: its register ops, then its write-back gates).
+// `factors`: diagonal ops on bits outside the tile become one tile factor
+// (a kind-2 phase; the specialised kernels apply it, the interpreter kernels
+// do not take such programs).
 void build_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R,
                    std::vector<SwPhase>& phases, std::vector<SwOp>& ops, bool search = true,
-                   std::vector<size_t>* order = nullptr);
+                   std::vector<size_t>* order = nullptr, bool factors = false);
 
-// The order in which the plain (per-gate) plan applies the queue's gates
-// (identity gates, which never run, are left out).
+// The order in which the plain (per-gate) plan of the specialised kernels
+// applies the queue's gates (identity gates, which never run, are left out;
+// tile factors last in their sweep).
 std::vector<size_t> plan_order(const std::vector<PendingGate>& q, int n, const TileGeom& g, bool search);
 
 // Fused program of one sweep: dense blocks of at most K local qubits
@@ -51,7 +55,7 @@ std::vector<size_t> plan_order(const std::vector<PendingGate>& q, int n, const T
 // register phases for the rest.
 void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step, int M, int R, int K,
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
-                         int* nblocks, bool search = true);
+                         int* nblocks, bool search = true, bool factors = false);
 
 // Features of a sweep program for the learned cost model (16 floats):
 // register phases, blocks of 3 and of 4 qubits (a block of k > 4 qubits

@@ -153,11 +153,11 @@ struct SweepProgram {
 
 // kind 0: per-gate register phases; 3 .. 6: dense blocks of that width
 static SweepProgram build_candidate(const std::vector<PendingGate>& q, const PlanStep& step, const TileGeom& g,
-                                    int kind, bool search)
+                                    int kind, bool search, bool factors)
 {
     SweepProgram p;
-    if (kind == 0) build_program(q, step, g.M, g.R, p.phases, p.ops, search);
-    else build_fused_program(q, step, g.M, g.R, kind, p.phases, p.ops, p.blocks, &p.nblocks, search);
+    if (kind == 0) build_program(q, step, g.M, g.R, p.phases, p.ops, search, nullptr, factors);
+    else build_fused_program(q, step, g.M, g.R, kind, p.phases, p.ops, p.blocks, &p.nblocks, search, factors);
     return p;
 }
 
@@ -180,10 +180,11 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     const TileGeom g = st.geometry();
     const int prec = int(st.precision());
     const bool search = st.plan_search(state_bytes);
+    const bool factors = st.jit_enabled();   // tile factors: specialised kernels only
     auto build_all = [&](const std::vector<PlanStep>& ss, int kind) {
         std::vector<SweepProgram> out(ss.size());
         for (size_t s = 0; s < ss.size(); ++s)
-            if (ss[s].sweep) out[s] = build_candidate(queue, ss[s], g, kind, search);
+            if (ss[s].sweep) out[s] = build_candidate(queue, ss[s], g, kind, search, factors);
         return out;
     };
     const bool fused = st.precision() == PREC_C64 && fusion_k > 0;
@@ -248,7 +249,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
             try {
                 for (size_t i = next++; i < work.size(); i = next++) {
                     const auto& w = work[i];
-                    cand[w[0]][w[1]][w[2]] = build_candidate(queue, tl[w[0]][w[1]], g, kinds[w[2]], search);
+                    cand[w[0]][w[1]][w[2]] = build_candidate(queue, tl[w[0]][w[1]], g, kinds[w[2]], search, factors);
                 }
             } catch (...) {
                 std::lock_guard<std::mutex> lk(err_mu);
@@ -331,7 +332,7 @@ std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int
     size_t nblk_floats = 0;
     for (const PlanStep& step : steps) {
         if (!step.sweep) continue;
-        SweepProgram p = build_candidate(q, step, g, K, search);
+        SweepProgram p = build_candidate(q, step, g, K, search, true);
         for (SwPhase& ph : p.phases)
             if (ph.kind == 1) ph.op_begin += uint32_t(nblk_floats);
         nblk_floats += p.blocks.size();
@@ -421,6 +422,9 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             int slot;
             // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
             const hipFunction_t jf = jitfn[s];
+            if (!jf)
+                for (const SwPhase& ph : progs[s].phases)
+                    if (ph.kind == 2) throw Error(ERR_HIP, "A sweep program with tile factors needs its specialised kernel");
             if (st.precision() == PREC_C128) {
                 if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf);
                 else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf);

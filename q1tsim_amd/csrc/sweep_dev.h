@@ -456,7 +456,9 @@ template <int M, int R> __device__ __forceinline__ void tile_addr(const SweepArg
     }
 }
 
-// The product, in program order, of the factors of the tile-factor ops
+// The specialised kernels' tile factors (a program's kind-2 phase; the
+// interpreter kernels' programs have none):
+// the product, in program order, of the factors of the tile-factor ops
 // [begin, end) for the tile at `base`: a phase-on-ones op contributes its
 // phase when all its (outer) bits are set, a one-target diagonal on an outer
 // bit d0 or d1 by that bit, each only when its outer controls are set.
@@ -483,18 +485,11 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
     constexpr int NT = 1 << (M - R), NR = 1 << R;
     using Rl = typename RealOf<V>::type;
     const int tid = threadIdx.x;
-    V fac = mkc<V>(1, 0);
-    bool scaled = false;
     {
         for (int p = 0; p < sa.nphases; ++p) {
             // the program is read through the constant address space: scalar
             // loads, whatever else the kernel writes (including the DMA)
             const SwPhase ph = load_uniform(sa.phases, uint32_t(p));
-            if (ph.kind == 2) {
-                fac = tile_factor<V>(sa.ops, ph.op_begin, ph.op_end, base);
-                scaled = true;
-                continue;
-            }
             if constexpr (sizeof(V) == 8) {
                 if (ph.kind == 1) {
                     // fused dense block: K = reg_mask, matrix at blocks + op_begin
@@ -584,13 +579,8 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
             __syncthreads();
         }
 
-        if (scaled) {
 #pragma unroll
-            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = cmul(fac, lds[swz<SW, M>(tid + k * NT)]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
-        }
+        for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
         __syncthreads();
     }
 }

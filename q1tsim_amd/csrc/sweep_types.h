@@ -50,7 +50,8 @@ struct SwPhase {
                          // 2: tile factors -- ops [op_begin, op_end) are
                          // diagonal ops on bits outside the tile only; the
                          // tile is scaled by the product of their factors
-                         // as it is stored (the program's last phase)
+                         // as it is stored (the program's last phase;
+                         // programs of the specialised kernels only)
     uint8_t rpos[8];     // local positions of the register bits (ascending)
     uint8_t tbit[16];    // local positions of the thread bits (ascending)
     // register phases: an affine map over GF(2) of the local index, applied

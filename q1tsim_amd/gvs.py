@@ -206,7 +206,11 @@ class GpuVectorState:
         check(lib().q1t_gvs_marginal_w0_noalloc(self.h, qbit, _dptr(out), len(out)))
         return out
 
-    def measure_into(self, qbit, cbit, res, rng, collapse=True):
+    def measure_into(self, qbit, cbit, res, rng, collapse=True, basis="Z"):
+        if basis != "Z":
+            check(lib().q1t_gvs_measure_basis_into(self.h, qbit, cbit, basis.encode(), _u64ptr(res), len(res),
+                                                   ctypes.byref(rng.rng), 1 if collapse else 0))
+            return
         f = lib().q1t_gvs_measure_into if collapse else lib().q1t_gvs_peek_into
         check(f(self.h, qbit, cbit, _u64ptr(res), len(res), ctypes.byref(rng.rng)))
 

@@ -37,7 +37,7 @@ def test_specialised_kernels_compile(precision, fusion_k, prefetch):
         for i in range(nk):
             co = os.path.join(d, "sweep_%d.co" % i)
             asm = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", co], capture_output=True, text=True).stdout
-            assert "q1t_jit_sweep" in asm
+            assert ("q1t_jit_sweep_c64" if precision == "c64" else "q1t_jit_sweep_c128") in asm
             assert not re.search(r"v_pk_(fma|mul|add)_f32", asm), "packed FP32 in %s" % co
             notes = subprocess.run([READELF, "--notes", co], capture_output=True, text=True).stdout
             m = re.search(r"\.private_segment_fixed_size:\s+(\d+)", notes)
