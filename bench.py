@@ -212,34 +212,32 @@ def batch_extra(count, seed=2024):
     return out
 
 
-def train_cost_model(record_n=28, seed=5):
-    """The learned planner's model, trained on sweeps the simulator records
-    itself: seeded exploration of the candidate programs (per-gate register
-    phases, 3- and 4-qubit blocks; both tilings) on complex64 states of
-    `record_n` qubits (2 GiB: tiles reach the high index bits the 30- and
-    32-qubit states use) running random-pair (depth 40), layered (depth 20)
-    and QFT circuits; minibatch Adam in the HIP kernels."""
-    from q1tsim_amd import costmodel, workloads
-    from q1tsim_amd.costmodel import CostModel
-    t0 = time.perf_counter()
-    circs = [workloads.random_pairs(record_n, 40, 1000 + k) for k in range(20)]
-    circs += [workloads.random_layered(record_n, 20, 2000 + k) for k in range(10)]
-    circs += [workloads.qft(record_n)] * 8   # explored anew on every flush
-    feat, tiles, ms = costmodel.record(record_n, circs, fusion_k=4, seed=seed)
-    rec_s = time.perf_counter() - t0
-    cm = CostModel(seed)
-    t0 = time.perf_counter()
-    mse = costmodel.fit(cm, feat, tiles, ms, epochs=40, batch=256)
-    info = {"record_n": record_n, "recorded_sweeps": int(len(feat)),
-            "recorded_blocks": int(feat[:, 1].sum() + feat[:, 2].sum()), "record_s": round(rec_s, 2),
-            "train_s": round(time.perf_counter() - t0, 2), "train_mse_log_time": round(mse, 4)}
-    return cm, info
+def cost_model(retrain):
+    """The learned planner's model: the shipped MI355X model
+    (q1tsim_amd/data/costmodel_gfx950.npz, written by
+    q1tsim_amd/train_costmodel.py from sweeps the simulator recorded on this
+    part: seeded exploration of the candidate programs on 28-qubit complex64
+    states, interpreter and specialised kernels, held-out circuits for
+    validation), or with `retrain` one trained in this run the same way."""
+    from q1tsim_amd import costmodel, train_costmodel
+    if retrain:
+        t0 = time.perf_counter()
+        cm, meta = train_costmodel.train(evaluate=False, log=lambda s: print(s, file=sys.stderr, flush=True))
+        meta["source"] = "trained in this run"
+        meta["seconds"] = round(time.perf_counter() - t0, 1)
+    else:
+        cm = costmodel.load()
+        meta = dict(cm.meta)
+        meta["source"] = "q1tsim_amd/data/costmodel_gfx950.npz"
+    keep = ("source", "n_qubits", "training_sweeps", "training_blocks", "train_mse_log_time", "held_out",
+            "held_out_circuits_specialised", "device", "seconds")
+    return cm, {k: meta[k] for k in keep if k in meta}
 
 
 def learned_extra(cm, info, n=32, depth=40, seed=5):
-    """Config: n-qubit depth-40 circuit with 25% CX on random pairs, complex64
-    with fused blocks: the learned plan against greedy fixed-width fusion on
-    a held-out circuit, both compared with the complex128 run."""
+    """Config: n-qubit depth-40 circuit with 25% CX on random pairs, complex64:
+    the learned plan against greedy fixed-width fusion and per-gate sweeps on
+    a held-out circuit, each compared with the complex128 run."""
     from q1tsim_amd import workloads
     from q1tsim_amd.gvs import GpuVectorState, compare
     ops = workloads.random_pairs(n, depth, seed)
@@ -251,10 +249,10 @@ def learned_extra(cm, info, n=32, depth=40, seed=5):
     apply_circuit(ref, ops)
     c128_s = time.perf_counter() - t0
     res = {"n_qubits": n, "depth": depth, "gates": len(ops), "c128_s": round(c128_s, 3)}
-    res.update(info)
-    for mode, name in ((0, "greedy"), (1, "learned")):
+    res["cost_model"] = info["source"]
+    for fk, mode, name in ((4, 0, "greedy"), (0, 0, "per_gate"), (4, 1, "learned")):
         s = GpuVectorState(n, 1, "c64")
-        s.set_plan(4, True)
+        s.set_plan(fk, True)
         s.set_cost_model(cm, mode, seed)
         s.set_jit(True)
         apply_circuit(s, ops)   # warm-up (compiles the specialised kernels)
@@ -268,6 +266,7 @@ def learned_extra(cm, info, n=32, depth=40, seed=5):
                      "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs}
         del s
     res["learned_gain"] = round(res["greedy"]["seconds"] / res["learned"]["seconds"], 4)
+    res["learned_vs_per_gate"] = round(res["per_gate"]["seconds"] / res["learned"]["seconds"], 4)
     return res
 
 
@@ -282,6 +281,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the QFT, sampling, batch and learned lines")
     ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--train-cost-model", action="store_true",
+                    help="train the learned planner's model in this run instead of loading the shipped one")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batch leg (both precisions), one JSON line; for kernel-trace profiles")
     ap.add_argument("--profile-workload", action="store_true",
@@ -355,7 +356,7 @@ def main():
     # FP32-MFMA blocks for k = 3..6, and the learned planner's choice per
     # sweep (register phases or 3-/4-qubit blocks), all on specialised
     # kernels; errors against the complex128 state
-    cm, cm_info = train_cost_model()
+    cm, cm_info = cost_model(args.train_cost_model)
     c64 = {}
     legs = [("per_gate", 0, 0)] + [("greedy_k%d" % k, k, 0) for k in (3, 4, 5, 6)] + [("learned", args.fusion_k, 1)]
     for name, fk, mode in legs:
@@ -441,7 +442,7 @@ def main():
                        "interpreter_ms": round(1e3 * float(np.median(t_int)), 3),
                        "interpreter_kernels": {k: {"launches": v[0], "ms": round(v[1], 3)}
                                                for k, v in prof_int.items()}},
-        "fused_c64": {"ms": ms_fused, "planner": "learned (cost model trained in this run)",
+        "fused_c64": {"ms": ms_fused, "planner": "learned (cost model: %s)" % cm_info["source"],
                       "sweeps": c64["learned"]["sweeps"], "blocks": c64["learned"]["blocks"],
                       "GBps": round(c64["learned"]["hbm_bytes"] / (ms_fused * 1e-3) / 1e9, 1),
                       "fusion_k": args.fusion_k, "speedup_vs_plain": round(ms_plain / ms_fused, 3),

@@ -85,7 +85,7 @@ int q1t_circuit_set_jit(circuit_t* ptr, int on);
 /* Build-added: plan a gate-only circuit on the host and compile the kernels
  * specialised to its sweeps with hipRTC (no GPU needed); precision as
  * q1t_circuit_set_backend, fusion_k > 0 greedy fusion, mode as
- * q1t_gvs_set_jit (1 or 2), geometry that of the specialised kernels.  With out_dir, each
+ * q1t_gvs_set_jit (1, 2 or 3), geometry that of the specialised kernels.  With out_dir, each
  * code object and source is written there as sweep_<i>.co / .hip. */
 int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int fusion_k, int mode, const char* out_dir,
                             size_t* nkernels, double* seconds);

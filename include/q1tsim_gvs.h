@@ -163,8 +163,12 @@ int q1t_gvs_set_graph(q1t_gvs* s, int on);
  * on the first flush that needs it and cached per process; the gate
  * matrices are still read from the uploaded program, so one kernel serves
  * every program of the same structure.  Same results as the interpreter
- * kernels (the same op bodies).  on = 2: the same kernels with each
- * workgroup's next tile loaded into registers while the current one runs. */
+ * kernels (the same op bodies).  Diagonal ops on bits outside a sweep's tile
+ * are folded into one factor per tile, applied as the tile is stored.
+ * on = 2: the same kernels with each workgroup's next tile loaded into
+ * registers while the current one runs.  on = 3: specialised kernels
+ * without tile factors -- every op where the interpreter applies it, so the
+ * amplitudes equal the interpreter kernels' bit for bit. */
 int q1t_gvs_set_jit(q1t_gvs* s, int on);
 /* Build-added: graphs captured and replays launched over the state's life. */
 int q1t_gvs_graph_stats(q1t_gvs* s, uint64_t* captures, uint64_t* replays);
@@ -218,8 +222,8 @@ uint64_t q1t_xoshiro_next(void* ctx);
  * predicts log(microseconds per 1024 tiles) of a sweep program.  Training
  * and inference run in HIP kernels; the model only ranks equivalent
  * programs, it never changes amplitudes. */
-#define Q1T_CM_FEATURES 16
-#define Q1T_CM_PARAMS 5313
+#define Q1T_CM_FEATURES 20
+#define Q1T_CM_PARAMS 5569
 typedef struct q1t_cost_model q1t_cost_model;
 int q1t_cm_new(uint64_t seed, q1t_cost_model** out);
 void q1t_cm_free(q1t_cost_model* cm);

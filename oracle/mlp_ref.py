@@ -2,10 +2,10 @@
 
 The reference has no learned component (its planner is q1tsim's per-gate
 loop, src/circuit.rs:643-762), so this restates the model's own mathematics:
-a 16 -> 64 -> 64 -> 1 multilayer perceptron with tanh hidden layers on
+a 20 -> 64 -> 64 -> 1 multilayer perceptron with tanh hidden layers on
 standardised features, the mean squared error against targets, its gradient
 by backpropagation, and one Adam step (Kingma & Ba 2015, Algorithm 1, with
-bias-corrected moments).  Parameter layout (one flat vector): W1[64][16],
+bias-corrected moments).  Parameter layout (one flat vector): W1[64][20],
 b1[64], W2[64][64], b2[64], w3[64], b3.
 
 The gradient is pinned in tests/test_oracle_mlp.py by central finite
@@ -16,7 +16,7 @@ Only ``tests/`` may import this module, and only as the checker.
 """
 import numpy as np
 
-F, H = 16, 64
+F, H = 20, 64
 W1 = 0
 B1 = W1 + H * F
 W2 = B1 + H
@@ -32,7 +32,7 @@ def unpack(p):
 
 
 def forward(p, x, mu, sigma):
-    """Outputs f(x) for rows x (n x 16), with the hidden activations."""
+    """Outputs f(x) for rows x (n x 20), with the hidden activations."""
     w1, b1, w2, b2, w3, b3 = unpack(p)
     xn = (np.asarray(x, dtype=np.float64) - mu) / sigma
     h1 = np.tanh(xn @ w1.T + b1)

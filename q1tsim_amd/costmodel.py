@@ -1,6 +1,6 @@
 """Learned sweep cost model (include/q1tsim_gvs.h, q1t_cm_*).
 
-A 16 -> 64 -> 64 -> 1 tanh MLP whose forward, backward and Adam kernels are
+A 20 -> 64 -> 64 -> 1 tanh MLP whose forward, backward and Adam kernels are
 hand-written HIP (csrc/costmodel.hip).  It predicts log(microseconds per
 1024 tiles) of a sweep program from the features csrc/plan.cpp computes
 (sweep_features); the fused complex64 planner uses it to rank equivalent
@@ -8,13 +8,17 @@ programs.  Training data are sweeps the simulator times itself: a state with
 recording on logs every sweep's features and HIP-event time.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 
 from ._lib import check, lib
 
-NFEAT = 16
-NPARAMS = 5313
+NFEAT = 20
+NPARAMS = 5569
+# the model trained on MI355X (gfx950) by train_costmodel.py
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "costmodel_gfx950.npz")
 
 
 def _fptr(a):
@@ -116,13 +120,17 @@ def fit(cm, feat, tiles, ms, epochs=60, batch=256, lr=2e-3, seed=1, min_steps=60
     return cm.train(feat, targets(tiles, ms), epochs=epochs, batch=batch, lr=lr, seed=seed)
 
 
-def record(nr_bits, circuits, fusion_k=4, seed=1, reps=1, geometry=None):
+def record(nr_bits, circuits, fusion_k=4, seed=1, reps=1, geometry=None, jit=False):
     """Run each circuit (a list of (name, qubits, params)) on a complex64
     state with the planner exploring its candidates, and return the recorded
-    sweeps (features, tiles, ms)."""
+    sweeps (features, tiles, ms).  `jit`: sweeps run by kernels specialised
+    to their programs (each new program is compiled before its flush runs;
+    the timings are the kernels' own HIP-event times)."""
     from .gvs import GpuVectorState, gate_matrix
     s = GpuVectorState(nr_bits, 1, "c64")
     s.set_plan(fusion_k, True)
+    if jit:
+        s.set_jit(True)
     if geometry is not None:
         s.set_geometry(*geometry)
     s.set_cost_model(None, 2, seed)
@@ -139,3 +147,44 @@ def record(nr_bits, circuits, fusion_k=4, seed=1, reps=1, geometry=None):
             ms.append(m)
     s.close()
     return np.concatenate(feats), np.concatenate(tiles), np.concatenate(ms)
+
+
+def errors(cm, feat, tiles, ms):
+    """Mean squared error of the predicted log time, and the mean and 90th
+    percentile of |predicted / measured - 1| in time."""
+    if len(feat) == 0:
+        return {"samples": 0}
+    y = cm.predict(feat).astype(np.float64)
+    t = targets(tiles, ms)
+    rel = np.abs(np.exp(y - t) - 1.0)
+    return {"samples": int(len(feat)), "mse_log_time": float(np.mean((y - t) ** 2)),
+            "mean_rel_err": float(rel.mean()), "p90_rel_err": float(np.quantile(rel, 0.9))}
+
+
+def save(cm, path, meta=None):
+    """Write the parameters, the feature normalisation and a JSON metadata
+    string (numpy .npz, no pickled objects)."""
+    mu, sg = cm.get_norm()
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    np.savez(path, params=cm.params, mu=mu, sigma=sg, nfeat=np.int32(NFEAT),
+             meta=np.array(json.dumps(meta or {})))
+
+
+def load(path=None, seed=1):
+    """The model stored by save() (default: the shipped MI355X model), marked
+    trained; raises when the file is missing or was written for another
+    feature count."""
+    path = path or DEFAULT_MODEL
+    with np.load(path, allow_pickle=False) as z:
+        if int(z["nfeat"]) != NFEAT or z["params"].shape != (NPARAMS,):
+            raise ValueError("%s: model of %d features / %d parameters, this library has %d / %d"
+                             % (path, int(z["nfeat"]), z["params"].shape[0], NFEAT, NPARAMS))
+        cm = CostModel(seed)
+        cm.params = z["params"]
+        cm.set_norm(z["mu"], z["sigma"])
+        meta = json.loads(str(z["meta"]))
+    cm.set_trained(True)
+    cm.meta = meta
+    return cm

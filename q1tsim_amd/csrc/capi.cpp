@@ -273,7 +273,7 @@ extern "C" int q1t_circuit_set_jit(circuit_t* ptr, int on)
 {
     NEED(ptr);
     return guard([&] {
-        if (on < 0 || on > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+        if (on < 0 || on > 3) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0..3");
         ptr->c.set_jit(on);
     });
 }
@@ -290,7 +290,7 @@ extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int 
     return guard([&] {
         if (precision != PREC_C128 && precision != PREC_C64) throw Error(ERR_INVALID_ARGUMENT, "unknown precision");
         if (fusion_k < 0 || fusion_k > 6) throw Error(ERR_INVALID_ARGUMENT, "fusion_k must be 0..6");
-        if (mode != 1 && mode != 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 1 or 2");
+        if (mode < 1 || mode > 3) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 1, 2 or 3");
         const size_t n = ptr->c.nr_qbits();
         std::vector<PendingGate> q;
         for (const CircuitOp& op : ptr->c.ops()) {
@@ -304,7 +304,8 @@ extern "C" int q1t_circuit_jit_compile(const circuit_t* ptr, int precision, int 
             for (size_t b : op.bits) pg.qbits.push_back(int(n - 1 - b));
             q.push_back(std::move(pg));
         }
-        const std::vector<std::string> srcs = plan_jit_sources(q, int(n), Precision(precision), fusion_k, true, mode == 2);
+        const std::vector<std::string> srcs =
+            plan_jit_sources(q, int(n), Precision(precision), fusion_k, true, mode == 2, mode != 3);
         double total = 0.0;
         for (size_t i = 0; i < srcs.size(); ++i) {
             double sec = 0.0;
@@ -798,7 +799,7 @@ extern "C" int q1t_gvs_set_jit(q1t_gvs* s, int on)
 {
     NEED(s);
     return guard([&] {
-        if (on < 0 || on > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+        if (on < 0 || on > 3) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0..3");
         s->s.set_jit(on);
     });
 }
@@ -945,6 +946,7 @@ struct q1t_cost_model {
 
 static_assert(kCmParams == Q1T_CM_PARAMS, "parameter count of the C ABI");
 static_assert(kCmFeat == Q1T_CM_FEATURES && kSweepFeatures == Q1T_CM_FEATURES, "feature count of the C ABI");
+static_assert(sizeof(SweepSample::feat) == kSweepFeatures * sizeof(float), "recorded sweep features");
 
 extern "C" int q1t_cm_new(uint64_t seed, q1t_cost_model** out)
 {

@@ -1,4 +1,4 @@
-// Learned sweep cost model: a small MLP (16 features -> 64 -> 64 -> 1, tanh,
+// Learned sweep cost model: a small MLP (20 features -> 64 -> 64 -> 1, tanh,
 // FP32) that predicts the log of a sweep's run time per 1024 tiles from
 // features of its program.  Inference and training (forward, backward,
 // gradient reduction, Adam) run in the HIP kernels of costmodel.hip.  The
@@ -14,7 +14,7 @@
 
 namespace q1t {
 
-constexpr int kCmFeat = 16;
+constexpr int kCmFeat = 20;
 constexpr int kCmHid = 64;
 // parameter layout (one float array): W1[kCmHid][kCmFeat], b1[kCmHid],
 // W2[kCmHid][kCmHid], b2[kCmHid], w3[kCmHid], b3

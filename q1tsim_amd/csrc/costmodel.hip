@@ -156,21 +156,24 @@ __global__ void __launch_bounds__(kCmRowsPerBlock) k_cm_grad(const float* __rest
         }
     }
     __syncthreads();
-    // W1[i][f] = sum_r d1[r][i] * x[r][f]: thread -> (i = tid / 2, 8 f's); b1
+    // W1[i][f] = sum_r d1[r][i] * x[r][f]: thread -> (i = tid / 2, half of
+    // the features); b1
     {
-        const int i = tid >> 1, f0 = (tid & 1) * 8;
-        float acc[8];
+        static_assert(kCmFeat % 2 == 0, "two threads per hidden unit split the features");
+        constexpr int FH = kCmFeat / 2;
+        const int i = tid >> 1, f0 = (tid & 1) * FH;
+        float acc[FH];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+        for (int k = 0; k < FH; ++k) acc[k] = 0.0f;
         float sb = 0.0f;
         for (int q = 0; q < NR; ++q) {
             const float di = ws[q][i];
             sb += di;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) acc[k] = fmaf(di, xs[q][f0 + k], acc[k]);
+            for (int k = 0; k < FH; ++k) acc[k] = fmaf(di, xs[q][f0 + k], acc[k]);
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) out[kCmW1 + i * kCmFeat + f0 + k] = acc[k];
+        for (int k = 0; k < FH; ++k) out[kCmW1 + i * kCmFeat + f0 + k] = acc[k];
         if ((tid & 1) == 0) out[kCmB1 + i] = sb;
     }
 }

@@ -57,7 +57,7 @@ enum PlanMode {
 
 // One recorded sweep: program features, tiles processed, measured time.
 struct SweepSample {
-    float feat[16];
+    float feat[20];       // sweep_features()
     double tiles = 0.0;   // tiles over all columns of the launch
     int slot = -1;        // profiling slot of the launch until its time is read
     double ms = 0.0;
@@ -260,6 +260,9 @@ public:
     void set_jit(int mode);
     bool jit_enabled() const { return jit_mode_ != 0; }
     bool jit_prefetch() const { return jit_mode_ == 2; }
+    // diagonal ops on bits outside the tile folded into one factor per tile
+    // (specialised kernels, modes 1 and 2)
+    bool tile_factors() const { return jit_mode_ == 1 || jit_mode_ == 2; }
 
     void set_graph(bool on);
     bool graph_enabled() const { return graph_on_ && !profiling_ && !recording_; }

@@ -714,7 +714,7 @@ void GpuVectorState::set_geometry(const TileGeom& g)
 
 void GpuVectorState::set_jit(int mode)
 {
-    if (mode < 0 || mode > 2) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) throw Error(ERR_INVALID_ARGUMENT, "jit mode must be 0..3");
     jit_mode_ = mode;
     if (!geom_set_) geom_ = mode ? tile_geometry_specialised(prec_) : tile_geometry(prec_);
 }

@@ -332,7 +332,7 @@ void JitCache::compile_all(const std::vector<std::string>& srcs)
     std::vector<std::vector<char>> codes(todo.size());
     std::vector<double> secs(todo.size(), 0.0);
     std::vector<std::string> errs(todo.size());
-    const size_t nth = std::min<size_t>(todo.size(), std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+    const size_t nth = std::min<size_t>(todo.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     std::vector<std::thread> pool;
     std::mutex qm;
     size_t next = 0;

@@ -801,7 +801,7 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
 }
 
 void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, const PlanStep& step,
-                    int precision, const TileGeom& g, float* f)
+                    int precision, const TileGeom& g, bool specialised, float* f)
 {
     for (int k = 0; k < kSweepFeatures; ++k) f[k] = 0.0f;
     for (const SwPhase& ph : phases) {
@@ -811,11 +811,14 @@ void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>&
             // blocks wider than 4 qubits count as 4-qubit blocks of equal work
             if (nb == 3) f[1] += 1.0f;
             else f[2] += float(1 << (2 * (nb - 4)));
+            f[16] = std::max(f[16], float(nb));
         } else if (ph.kind == 0) {
             nb = g.R;
             f[0] += 1.0f;
         } else {
-            continue;   // the tile factors: no LDS exchange
+            // the tile factors: no LDS exchange, one multiply per amplitude
+            f[18] += float(ph.op_end - ph.op_begin);
+            continue;
         }
         for (int j = 0; j < nb; ++j)
             if (ph.rpos[j] >= 8) {
@@ -838,9 +841,12 @@ void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>&
     f[10] = float(precision);
     f[11] = float(g.M);
     f[12] = float(g.R);
-    for (int b : step.tile)
+    for (int b : step.tile) {
         if (b >= 20) f[14] += 1.0f;
+        if (b >= 26) f[17] += 1.0f;
+    }
     f[15] = float(step.gates.size());
+    f[19] = specialised ? 1.0f : 0.0f;
 }
 
 }  // namespace q1t

@@ -57,15 +57,18 @@ void build_fused_program(const std::vector<PendingGate>& q, const PlanStep& step
                          std::vector<SwPhase>& phases, std::vector<SwOp>& ops, std::vector<float>& blocks,
                          int* nblocks, bool search = true, bool factors = false);
 
-// Features of a sweep program for the learned cost model (16 floats):
-// register phases, blocks of 3 and of 4 qubits (a block of k > 4 qubits
-// counts as 4^(k-4) four-qubit blocks), one-qubit ops by body
-// (general, real/RX/Y, swap, phase/diagonal), ops with thread, register and
-// outer controls, precision, M, R, phases holding a local bit >= 8 in
-// registers, tile bits at positions >= 20 (long HBM strides), gates.
-constexpr int kSweepFeatures = 16;
+// Features of a sweep program for the learned cost model (20 floats):
+//  0 register phases, 1 blocks of 3 qubits, 2 blocks of 4 qubits (a block of
+//  k > 4 qubits counts as 4^(k-4) four-qubit blocks), 3-6 one-qubit ops by
+//  body (general, real/RX/Y, swap, phase/diagonal), 7-9 ops with thread,
+//  register and outer controls, 10 precision, 11 M, 12 R, 13 phases holding
+//  a local bit >= 8 in registers, 14 tile bits at positions >= 20 (long HBM
+//  strides), 15 gates, 16 widest block, 17 tile bits at positions >= 26,
+//  18 ops folded into the tile factors, 19 kernel specialised to the
+//  program (1) or the interpreter (0).
+constexpr int kSweepFeatures = 20;
 void sweep_features(const std::vector<SwPhase>& phases, const std::vector<SwOp>& ops, const PlanStep& step,
-                    int precision, const TileGeom& g, float* out);
+                    int precision, const TileGeom& g, bool specialised, float* out);
 
 // Device buffer that grows on demand (synchronising `stream` first).
 struct DeviceBuffer {
@@ -85,9 +88,10 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
 
 // Sources of the kernels specialised to the sweeps of the plan a flush of
 // `q` would run (plain programs; with fusion_k > 0 greedy fixed-width
-// fusion), block offsets as in the flush's merged table.  Host only.
+// fusion), block offsets as in the flush's merged table; `factors`: diagonal
+// ops on bits outside the tile folded into tile factors.  Host only.
 std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int n, Precision prec, int fusion_k,
-                                          bool search, bool prefetch = false);
+                                          bool search, bool prefetch = false, bool factors = true);
 
 // Apply the pending gates to columns [col_begin, col_end) of the state with
 // the planned schedule: runs of gates whose qubits fit one tile are applied

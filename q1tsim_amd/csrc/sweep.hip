@@ -180,7 +180,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     const TileGeom g = st.geometry();
     const int prec = int(st.precision());
     const bool search = st.plan_search(state_bytes);
-    const bool factors = st.jit_enabled();   // tile factors: specialised kernels only
+    const bool factors = st.tile_factors();   // specialised kernels only
     auto build_all = [&](const std::vector<PlanStep>& ss, int kind) {
         std::vector<SweepProgram> out(ss.size());
         for (size_t s = 0; s < ss.size(); ++s)
@@ -283,7 +283,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
             for (size_t s = 0; s < tl[t].size(); ++s)
                 if (tl[t][s].sweep)
                     for (int c = 0; c < nk; ++c) {
-                        sweep_features(cand[t][s][c].phases, cand[t][s][c].ops, tl[t][s], prec, g, f);
+                        sweep_features(cand[t][s][c].phases, cand[t][s][c].ops, tl[t][s], prec, g, st.jit_enabled(), f);
                         feat.insert(feat.end(), f, f + kSweepFeatures);
                         who.push_back(Who{t, s, c});
                     }
@@ -323,7 +323,7 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
 }
 
 std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int n, Precision prec, int fusion_k,
-                                          bool search, bool prefetch)
+                                          bool search, bool prefetch, bool factors)
 {
     const TileGeom g = tile_geometry_specialised(prec);
     const int K = prec == PREC_C64 && fusion_k > 0 ? std::min(6, std::max(3, fusion_k)) : 0;
@@ -332,7 +332,7 @@ std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int
     size_t nblk_floats = 0;
     for (const PlanStep& step : steps) {
         if (!step.sweep) continue;
-        SweepProgram p = build_candidate(q, step, g, K, search, true);
+        SweepProgram p = build_candidate(q, step, g, K, search, factors);
         for (SwPhase& ph : p.phases)
             if (ph.kind == 1) ph.op_begin += uint32_t(nblk_floats);
         nblk_floats += p.blocks.size();
@@ -435,7 +435,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             }
             if (st.recording() && col_end - col_begin <= 65535) {
                 float f[kSweepFeatures];
-                sweep_features(progs[s].phases, progs[s].ops, step, int(st.precision()), g, f);
+                sweep_features(progs[s].phases, progs[s].ops, step, int(st.precision()), g, jf != nullptr, f);
                 st.add_sample(f, tiles, slot);
             }
             stats.launches += 1;

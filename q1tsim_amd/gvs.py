@@ -93,6 +93,18 @@ class Xoshiro:
             self.h = None
 
 
+def _jit_mode(on, prefetch, factors):
+    """The C ABI's specialised-kernel mode: 0 off, 1 on, 2 on with prefetch,
+    3 on without tile factors."""
+    if not on:
+        return 0
+    if not factors:
+        if prefetch:
+            raise ValueError("prefetching kernels always fold tile factors")
+        return 3
+    return 2 if prefetch else 1
+
+
 class GpuVectorState:
     def __init__(self, nr_bits, nr_shots, precision="c128"):
         h = ctypes.c_void_p()
@@ -142,11 +154,14 @@ class GpuVectorState:
         uploaded before each replay)."""
         check(lib().q1t_gvs_set_graph(self.h, int(on)))
 
-    def set_jit(self, on=True, prefetch=False):
+    def set_jit(self, on=True, prefetch=False, factors=True):
         """Sweep kernels specialised to each sweep program (compiled with
         hipRTC on first use, cached per process by program structure);
-        `prefetch`: the next tile is loaded while the current one runs."""
-        check(lib().q1t_gvs_set_jit(self.h, (2 if prefetch else 1) if on else 0))
+        `prefetch`: the next tile is loaded while the current one runs;
+        `factors`: diagonal ops on bits outside the tile folded into one
+        factor per tile (without them the amplitudes equal the interpreter
+        kernels' bit for bit)."""
+        check(lib().q1t_gvs_set_jit(self.h, _jit_mode(on, prefetch, factors)))
 
     def graph_stats(self):
         """(graphs captured, replays launched) over the state's life."""
@@ -272,8 +287,9 @@ class GpuVectorState:
         check(lib().q1t_gvs_set_recording(self.h, 1 if on else 0))
 
     def take_samples(self, cap=1 << 20):
-        """Recorded sweeps: (features n x 16 float32, tiles n, ms n)."""
-        feat = np.zeros((cap, 16), dtype=np.float32)
+        """Recorded sweeps: (features n x 20 float32, tiles n, ms n)."""
+        from .costmodel import NFEAT
+        feat = np.zeros((cap, NFEAT), dtype=np.float32)
         tiles = np.zeros(cap, dtype=np.float64)
         ms = np.zeros(cap, dtype=np.float64)
         n = ctypes.c_size_t(0)

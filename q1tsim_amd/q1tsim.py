@@ -267,16 +267,20 @@ class Circuit(object):
         previous execute's, with reference parameters re-read."""
         _lib.check(self.__sim.q1t_circuit_set_graph(self.__ptr, 1 if on else 0))
 
-    def set_jit(self, on=True, prefetch=False):
-        """Sweep kernels specialised to the state's sweep programs."""
-        _lib.check(self.__sim.q1t_circuit_set_jit(self.__ptr, (2 if prefetch else 1) if on else 0))
+    def set_jit(self, on=True, prefetch=False, factors=True):
+        """Sweep kernels specialised to the state's sweep programs (modes as
+        GpuVectorState.set_jit)."""
+        from .gvs import _jit_mode
+        _lib.check(self.__sim.q1t_circuit_set_jit(self.__ptr, _jit_mode(on, prefetch, factors)))
 
-    def jit_compile(self, precision="c128", fusion_k=0, out_dir=None, prefetch=False):
+    def jit_compile(self, precision="c128", fusion_k=0, out_dir=None, prefetch=False, factors=True):
         """Plan this gate-only circuit on the host and compile the kernels
         specialised to its sweeps (no GPU needed): (kernels, seconds)."""
+        from .gvs import _jit_mode
         prec = {"c128": 0, "c64": 1}[precision]
         n, sec = ctypes.c_size_t(0), ctypes.c_double(0.0)
-        _lib.check(self.__sim.q1t_circuit_jit_compile(self.__ptr, prec, int(fusion_k), 2 if prefetch else 1,
+        _lib.check(self.__sim.q1t_circuit_jit_compile(self.__ptr, prec, int(fusion_k),
+                                                      _jit_mode(True, prefetch, factors),
                                                       out_dir.encode() if out_dir else None,
                                                       ctypes.byref(n), ctypes.byref(sec)))
         return int(n.value), float(sec.value)

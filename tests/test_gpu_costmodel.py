@@ -27,10 +27,10 @@ def random_model(seed):
     p[M.B2:M.W3] = rng.uniform(-0.3, 0.3, M.H)
     p[M.B3] = 0.25
     cm.params = p
-    mu = rng.normal(size=16).astype(np.float32)
-    sg = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    mu = rng.normal(size=M.F).astype(np.float32)
+    sg = rng.uniform(0.5, 2.0, M.F).astype(np.float32)
     cm.set_norm(mu, sg)
-    x = (rng.normal(size=(300, 16)) * 2 + 1).astype(np.float32)   # 300 rows: ragged last workgroup
+    x = (rng.normal(size=(300, M.F)) * 2 + 1).astype(np.float32)   # 300 rows: ragged last workgroup
     t = rng.normal(size=300).astype(np.float32)
     return cm, cm.params.astype(np.float64), mu.astype(np.float64), sg.astype(np.float64), x, t
 
@@ -68,10 +68,10 @@ def test_adam_steps_match_oracle():
 
 def test_training_fits_known_function():
     rng = np.random.default_rng(4)
-    x = rng.uniform(-2, 2, size=(4096, 16)).astype(np.float32)
+    x = rng.uniform(-2, 2, size=(4096, M.F)).astype(np.float32)
     t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
     cm = CostModel(7)
-    cm.set_norm(np.zeros(16), np.ones(16))
+    cm.set_norm(np.zeros(M.F), np.ones(M.F))
     l0, _ = cm.loss_grad(x, t)
     loss = cm.train(x, t, epochs=150, batch=256, lr=3e-3, seed=1)
     assert loss < 0.05 * l0, (l0, loss)
@@ -120,40 +120,57 @@ def test_planner_modes_keep_amplitudes(trained, mode):
     assert infid <= 1e-6 and maxabs <= 1e-5, (mode, infid, maxabs)
 
 
-def test_learned_plan_vs_greedy_walltime():
-    """What the trained network gains over the method without it: wall time
-    (host planning included, median of 3 flushes) of the learned plan against
-    greedy fixed-width fusion on a held-out 28-qubit random-pair circuit,
-    with a model recorded on 26-qubit states; written to gpurun_out."""
-    n, rec_n = 28, 26
-    circs = [workloads.random_pairs(rec_n, 40, 500 + k) for k in range(8)]
-    circs += [workloads.random_layered(rec_n, 12, 600 + k) for k in range(4)]
-    feat, tiles, ms = costmodel.record(rec_n, circs, fusion_k=4, seed=3)
-    cm = CostModel(5)
-    loss = costmodel.fit(cm, feat, tiles, ms, epochs=40)
-    ops = workloads.random_pairs(n, 40, 12345)
-    res = {}
-    for mode in (0, 1):
-        s = GpuVectorState(n, 1, "c64")
-        s.set_plan(4, True)
-        s.set_cost_model(cm, mode, 1)
-        ts = []
-        for _ in range(3):
-            for g, b, p in ops:
-                s.apply_gate(gate_matrix(g, p), b, g)
-            t0 = time.perf_counter()
-            s.flush()
+def flush_seconds(ops, n, cm, fk, mode, reps=3):
+    """Median wall time (host planning included) of the circuit's flush on a
+    complex64 state with specialised kernels, after one flush that compiles
+    them; and the state's per-flush sweep and block counts."""
+    s = GpuVectorState(n, 1, "c64")
+    s.set_plan(fk, True)
+    s.set_cost_model(cm, mode, 1)
+    s.set_jit(True)
+    ts = []
+    for r in range(reps + 1):
+        for g, b, p in ops:
+            s.apply_gate(gate_matrix(g, p), b, g)
+        t0 = time.perf_counter()
+        s.flush()
+        if r:
             ts.append(time.perf_counter() - t0)
-        res[mode] = (float(np.median(ts)), s.stats())
-        s.close()
-    greedy, learned = res[0][0], res[1][0]
+    st = s.stats()
+    s.close()
+    return float(np.median(ts)), st["sweeps"] // (reps + 1), st["blocks"] // (reps + 1)
+
+
+def test_shipped_model_loads_and_validates():
+    cm = costmodel.load()
+    ho = cm.meta["held_out"]
+    for kind in ("interpreter", "specialised"):
+        assert ho[kind]["samples"] > 0 and ho[kind]["mean_rel_err"] < 0.25, (kind, ho[kind])
+    # the loaded parameters are the ones the oracle network evaluates
+    with np.load(costmodel.DEFAULT_MODEL, allow_pickle=False) as z:
+        p, mu, sg = z["params"], z["mu"], z["sigma"]
+    x = (mu + sg * np.random.default_rng(3).normal(size=(64, M.F))).astype(np.float32)
+    yr, _ = M.forward(p.astype(np.float64), x.astype(np.float64), mu.astype(np.float64), sg.astype(np.float64))
+    assert np.max(np.abs(cm.predict(x) - yr)) <= 1e-4 * (1 + np.max(np.abs(yr)))
+
+
+def test_learned_plan_vs_greedy_walltime():
+    """What the trained network gains over the method without it, on a
+    28-qubit random-pair circuit held out of its training (the shipped model,
+    recorded on 28-qubit states of other circuits): the learned plan's wall
+    time (host planning included, median of 3 flushes, specialised kernels)
+    is at least 5 % below greedy fixed-width fusion's and within 10 % of the
+    per-gate plan's; written to gpurun_out."""
+    n = 28
+    cm = costmodel.load()
+    ops = workloads.random_pairs(n, 40, 12345)
+    res = {name: flush_seconds(ops, n, cm, fk, mode)
+           for name, fk, mode in (("greedy", 4, 0), ("per_gate", 0, 0), ("learned", 4, 1))}
     os.makedirs("gpurun_out", exist_ok=True)
     with open("gpurun_out/learned_vs_greedy_n%d.json" % n, "w") as f:
-        json.dump({"n": n, "record_n": rec_n, "samples": int(len(feat)), "train_mse": loss, "greedy_s": greedy,
-                   "learned_s": learned, "greedy_stats": res[0][1], "learned_stats": res[1][1]}, f)
-    print("learned/greedy wall time: %.4f / %.4f s" % (learned, greedy))
-    # the learned plan moves less data (fewer sweeps) ...
-    assert res[1][1]["sweeps"] <= res[0][1]["sweeps"]
-    # ... and is not slower beyond timing noise (5 %); its gain at 32 qubits,
-    # where the kernels dwarf planning, is in the bench line
-    assert learned <= 1.05 * greedy
+        json.dump({"n": n, "held_out": cm.meta["held_out"],
+                   "plans": {k: {"seconds": v[0], "sweeps": v[1], "blocks": v[2]} for k, v in res.items()}}, f)
+    greedy, per_gate, learned = res["greedy"][0], res["per_gate"][0], res["learned"][0]
+    print("learned / greedy / per-gate wall time: %.4f / %.4f / %.4f s" % (learned, greedy, per_gate))
+    assert learned <= 0.95 * greedy
+    assert learned <= 1.10 * per_gate

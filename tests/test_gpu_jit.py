@@ -22,13 +22,13 @@ def mixed(n, seed):
     return ops
 
 
-def run(n, ops, precision, fusion_k, jit, cols=1, prefetch=False, geometry=None):
+def run(n, ops, precision, fusion_k, jit, cols=1, prefetch=False, geometry=None, factors=True):
     s = GpuVectorState(n, max(cols, 1), precision)
     s.set_plan(fusion_k, True)
     s.set_plan_search(1)
     if geometry:
         s.set_geometry(*geometry)
-    s.set_jit(jit, prefetch)
+    s.set_jit(jit, prefetch, factors)
     if cols > 1:   # several branch columns: measure qubits in |+>
         for q in range(int(np.log2(cols))):
             s.apply_gate(gate_matrix("h", []), [q])
@@ -46,25 +46,37 @@ def run(n, ops, precision, fusion_k, jit, cols=1, prefetch=False, geometry=None)
     return out, st
 
 
-@pytest.mark.parametrize("prefetch", [False, True])
 @pytest.mark.parametrize("precision,fusion_k", [("c128", 0), ("c64", 0), ("c64", 4)])
 @pytest.mark.parametrize("n", [14, 17])
-def test_jit_equals_interpreter(precision, fusion_k, n, prefetch):
+def test_jit_equals_interpreter(precision, fusion_k, n):
+    """Without tile factors the specialised kernels run the interpreter's
+    program with the same op bodies: equal amplitudes bit for bit.  With
+    them (the default) diagonal ops outside the tile become one product per
+    tile, a different rounding order: equal to the interpreter within a few
+    ulps, and the prefetching kernels equal the plain ones bit for bit."""
     ops = mixed(n, 40 + n)
     c0 = jit_stats()[0]
-    # the same tile shape for both (the specialised kernels' default differs)
+    # the same tile shape for all (the specialised kernels' default differs)
     geom = (12, 4, 4)
     a, st_a = run(n, ops, precision, fusion_k, False, geometry=geom)
-    b, st_b = run(n, ops, precision, fusion_k, True, prefetch=prefetch, geometry=geom)
+    b, st_b = run(n, ops, precision, fusion_k, True, geometry=geom, factors=False)
     assert st_a["sweeps"] == st_b["sweeps"] and st_b["sweeps"] > 0
     assert jit_stats()[0] >= c0
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+    c, st_c = run(n, ops, precision, fusion_k, True, geometry=geom)
+    d, _ = run(n, ops, precision, fusion_k, True, geometry=geom, prefetch=True)
+    assert st_c["sweeps"] == st_a["sweeps"]
+    tol = 1e-13 if precision == "c128" else 2e-6
+    for x, y, z in zip(a, c, d):
+        assert np.abs(x - y).max() <= tol
+        assert np.array_equal(y, z)
     if precision == "c128":
         r = orc.RefState(n, 1)
         for g, bb, p in ops:
             r.apply_gate(g, bb, p)
         assert np.abs(b[0] - r.states[:, 0]).max() <= 1e-12
+        assert np.abs(c[0] - r.states[:, 0]).max() <= 1e-12
 
 
 def test_jit_several_columns_and_wide_blocks():

@@ -54,3 +54,30 @@ def test_jit_compile_rejects_non_gate_ops():
     c.measure(0, 0)
     with pytest.raises(Exception, match="non-gate ops"):
         c.jit_compile("c128")
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_tile_factors_only_when_asked(precision):
+    """Diagonal gates on qubits outside the tile of a 20-qubit state (the
+    tile holds the dense gates' bits and the lowest bits) become tile
+    factors in the default specialised kernels and stay ops without them
+    (mode 3)."""
+    n = 20
+    c = Circuit(n, 0)
+    for q in range(8):
+        c.add_gate("h", [q], None)
+    for q in range(8, 14):
+        c.add_gate("rz", [q], [0.1 * q])
+        c.add_gate("t", [q], None)
+    for q in range(8):
+        c.add_gate("rx", [q], [0.2])
+    with tempfile.TemporaryDirectory() as d:
+        nk, _ = c.jit_compile(precision, 0, d)
+        srcs = [open(os.path.join(d, "sweep_%d.hip" % i)).read() for i in range(nk)]
+        assert sum(s.count("fac = cmul(fac") for s in srcs) == 12
+    with tempfile.TemporaryDirectory() as d:
+        nk, _ = c.jit_compile(precision, 0, d, factors=False)
+        srcs = [open(os.path.join(d, "sweep_%d.hip" % i)).read() for i in range(nk)]
+        assert not any("fac = cmul(fac" in s for s in srcs)
+    with pytest.raises(ValueError):
+        c.jit_compile(precision, 0, None, prefetch=True, factors=False)
