@@ -204,6 +204,8 @@ def batch_extra(count, seed=2024):
         out[prec] = {"seconds": round(dt, 3), "circuits_per_s": round(count / dt, 1), "warmup_runs": 1,
                      "waves": st["waves"],
                      "rounds": st["rounds"], "sync_ops": st["sync_ops"], "gate_launches": st["launches"],
+                     "column_launches": st["column_launches"], "solo_ops": st["solo_ops"],
+                     "plan_s": round(st["plan_s"], 3), "pack_s": round(st["pack_s"], 3),
                      "gates": st["gates"], "sweeps": st["sweeps"], "sweep_hbm_bytes": st["bytes"],
                      "split_s": {k: round(st[k], 3) for k in ("start_s", "queue_s", "flush_s", "measure_s", "other_s")},
                      "autotune": {"seconds": round(tune_s, 3), "chosen_M_R_L_fusion": list(b.config()),

@@ -129,6 +129,11 @@ typedef struct q1t_batch_stats {
     double start_s, queue_s, flush_s, measure_s, other_s;   /* its parts: state set-up, gate queueing,
                                                                batched flushes, batched measurements,
                                                                other ops (conditional gates, resets) */
+    uint64_t column_launches;   /* kernels and copies of the batched column work (measurements,
+                                   collapses, conditional gates, basis columns) */
+    uint64_t solo_ops;          /* ops run circuit by circuit (outside the batched kinds) */
+    double plan_s;              /* host planning inside the batched flushes */
+    double pack_s;              /* host table building and upload inside the batched flushes */
 } q1t_batch_stats;
 /* mem_budget: device bytes one wave of states may take (0: 3/4 of free memory) */
 int q1t_batch_new(int precision, int fusion_k, size_t mem_budget, q1t_batch** out);

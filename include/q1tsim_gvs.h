@@ -192,8 +192,8 @@ int q1t_gvs_kernel_profile(const q1t_gvs* s, q1t_kernel_profile* out, size_t cap
  * `bytes` (HIP events), i.e. the read+write streaming rate of the part. */
 int q1t_copy_bandwidth(size_t bytes, int reps, double* best_ms);
 /* Build-added, no counterpart in the reference: the process-wide cache of
- * released device blocks (state buffers are reused by exact size).  Limit
- * in bytes (0: half of device memory, the default); lowering it frees the
+ * released device blocks (state buffers are reused by size class).  Limit
+ * in bytes (0: 7/8 of device memory, the default); lowering it frees the
  * oldest cached blocks at once.  A host that shares the GPU with other
  * allocators lowers it. */
 int q1t_device_cache_limit(size_t bytes);

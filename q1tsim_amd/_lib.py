@@ -40,7 +40,8 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("gates", Stats), ("waves", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
                 ("sync_ops", ctypes.c_uint64), ("seconds", ctypes.c_double), ("start_s", ctypes.c_double),
                 ("queue_s", ctypes.c_double), ("flush_s", ctypes.c_double), ("measure_s", ctypes.c_double),
-                ("other_s", ctypes.c_double)]
+                ("other_s", ctypes.c_double), ("column_launches", ctypes.c_uint64), ("solo_ops", ctypes.c_uint64),
+                ("plan_s", ctypes.c_double), ("pack_s", ctypes.c_double)]
 
 
 class BatchConfig(ctypes.Structure):

@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics", "-mllvm", "-simplifycfg-sink-common=false",
           "-mllvm", "-structurizecfg-skip-uniform-regions"]
 
-SOURCES = ["gvs.hip", "sweep.hip", "costmodel.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp",
+SOURCES = ["gvs.hip", "items.hip", "sweep.hip", "costmodel.hip", "gates.cpp", "plan.cpp", "circuit.cpp", "batch.cpp",
            "capi.cpp", "jit.cpp"]
 
 # Device headers the run-time specialised sweep kernels include (jit.cpp

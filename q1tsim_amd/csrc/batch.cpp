@@ -2,7 +2,11 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 namespace q1t {
 
@@ -13,24 +17,12 @@ Batch::Batch(Precision prec, int fusion_k, size_t mem_budget)
 
 Batch::~Batch()
 {
-    if (host_) (void)hipHostFree(host_);
-}
-
-char* Batch::pinned(size_t n)
-{
-    if (n > host_cap_) {
-        if (host_) {
-            if (stream_) (void)hipStreamSynchronize(stream_->s);
-            (void)hipHostFree(host_);
-            host_ = nullptr;
-            host_cap_ = 0;
+    if (iq_) {
+        try {
+            ItemQueuePool::give(std::move(iq_));
+        } catch (...) {
         }
-        const size_t cap = std::max<size_t>(n, size_t(1) << 20);
-        if (hipHostMalloc(reinterpret_cast<void**>(&host_), cap, hipHostMallocDefault) != hipSuccess)
-            throw Error(ERR_OUT_OF_MEMORY, "Pinned host allocation failed");
-        host_cap_ = cap;
     }
-    return host_;
 }
 
 void Batch::set_config(const TileGeom& g, int fusion_k)
@@ -153,7 +145,7 @@ size_t Batch::resident_bytes(const Circuit& c, size_t nr_shots) const
         peak = std::max(peak, cols);
     }
     const size_t elem = prec_ == PREC_C128 ? 16 : 8;
-    return peak * (size_t(1) << c.nr_qbits()) * elem;
+    return DeviceCache::size_class(peak * (size_t(1) << c.nr_qbits()) * elem);
 }
 
 void Batch::execute(size_t nr_shots, bool keep_states)
@@ -164,9 +156,84 @@ void Batch::execute(size_t nr_shots, bool keep_states)
     stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Queue the gate ops of every circuit up to its next non-gate op, on host
+// threads (each circuit builds its own matrices and analyses its own gates).
+static std::vector<char> queue_all(const std::vector<Circuit*>& active)
+{
+    std::vector<char> more(active.size(), 0);
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nth = active.size() < 64 ? 1 : std::min(hw, active.size() / 32);
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&] {
+        try {
+            for (size_t k = next++; k < active.size(); k = next++) more[k] = active[k]->queue_gates() ? 1 : 0;
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = std::current_exception();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nth; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+    if (err) std::rethrow_exception(err);
+    return more;
+}
+
+// The non-gate ops at the circuits' cursors, until every circuit is at a
+// gate op or at its end.  Each pass takes one op of every circuit:
+// one-qubit measurements, Z-basis measure_all and conditional gates of all
+// circuits as items of one queue (one launch per kind, one synchronisation,
+// then every circuit's draws from its own random stream and one launch of
+// the collapses); any other op circuit by circuit.
+void Batch::sync_ops(std::vector<Circuit*>& active)
+{
+    ItemQueue& q = *iq_;
+    auto tick = std::chrono::steady_clock::now();
+    auto lap = [&](double& acc) {
+        const auto now = std::chrono::steady_clock::now();
+        acc += std::chrono::duration<double>(now - tick).count();
+        tick = now;
+    };
+    for (;;) {
+        std::vector<Circuit*> meas, all, solo;
+        bool any = false;
+        for (Circuit* c : active) {
+            if (c->at_end() || c->next_is_gate()) continue;
+            any = true;
+            if (c->next_is_measure()) meas.push_back(c);
+            else if (c->next_is_z_measure_all()) all.push_back(c);
+            else if (!(c->next_is_conditional() && c->conditional_enqueue(q))) solo.push_back(c);
+        }
+        if (!any) break;
+        for (Circuit* c : meas) c->measure_enqueue(q);
+        for (Circuit* c : all) c->measure_all_enqueue(q);
+        const uint64_t l0 = q.launches();
+        if (!q.empty()) q.launch(stream_->s);
+        if (!meas.empty() || !all.empty()) {
+            if (hipStreamSynchronize(stream_->s) != hipSuccess) throw Error(ERR_HIP, "measurement launch failed");
+            for (Circuit* c : meas) c->measure_complete(q);
+            for (Circuit* c : all) c->measure_all_complete(q);
+            if (!q.empty()) q.launch(stream_->s);
+        }
+        stats_.column_launches += q.launches() - l0;
+        stats_.sync_ops += meas.size() + all.size();
+        lap(stats_.measure_s);
+        for (Circuit* c : solo) {
+            c->run_next_op();
+            stats_.sync_ops += 1;
+            stats_.solo_ops += 1;
+        }
+        lap(stats_.other_s);
+    }
+}
+
 void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_states)
 {
     if (!stream_) stream_ = std::make_shared<SharedStream>();
+    if (!iq_) iq_ = ItemQueuePool::take(prec_ == PREC_C64);
     size_t budget = mem_budget_;
     if (!budget) {
         size_t free_b = 0, total_b = 0;
@@ -193,73 +260,37 @@ void Batch::run(const std::vector<Circuit*>& list, size_t nr_shots, bool keep_st
             acc += std::chrono::duration<double>(now - tick).count();
             tick = now;
         };
+        // states start as compact |0..0> columns: written out by the first
+        // flush, all of them in one launch
         for (Circuit* c : wave) {
-            c->start(nr_shots, stream_);
+            c->start(nr_shots, stream_, true);
             c->state()->set_geometry(geom_);
         }
         lap(stats_.start_s);
         std::vector<Circuit*> active = wave;
         std::vector<GpuVectorState*> states;
-        std::vector<char> more;
         while (!active.empty()) {
+            const std::vector<char> more = queue_all(active);
             states.clear();
-            more.assign(active.size(), 0);
-            for (size_t k = 0; k < active.size(); ++k) {
-                more[k] = active[k]->queue_gates() ? 1 : 0;
-                states.push_back(active[k]->state());
-            }
+            for (Circuit* c : active) states.push_back(c->state());
             lap(stats_.queue_s);
-            run_planned_batch(states, stream_->s, prog_, stats_.gates, tile_chunk_);
+            const uint64_t l0 = iq_->launches();
+            run_planned_batch(states, stream_->s, prog_, stage_, *iq_, stats_.gates, tile_chunk_, &stats_.plan_s,
+                              &stats_.pack_s);
+            stats_.column_launches += iq_->launches() - l0;
             stats_.rounds += 1;
             lap(stats_.flush_s);
-            // Z-basis measurements of all circuits (single-qubit marginals
-            // and measure_all sampling): everything enqueued, one
-            // synchronisation, then each circuit's draws (its own random
-            // stream), collapse and bit mapping
-            std::vector<Circuit*> still, meas;
-            std::vector<size_t> off;
-            std::vector<char> kind;   // 1: measure / peek, 2: measure_all / peek_all
-            std::vector<char> measured(active.size(), 0);
-            size_t total = 0;
-            for (size_t k = 0; k < active.size(); ++k) {
-                if (!more[k]) continue;
-                Circuit* c = active[k];
-                const char kd = c->next_is_measure() ? 1 : c->next_is_z_measure_all() ? 2 : 0;
-                if (!kd) continue;
-                measured[k] = 1;
-                meas.push_back(c);
-                kind.push_back(kd);
-                off.push_back(total);
-                const size_t b = kd == 1 ? c->measure_partials() * sizeof(double) : c->measure_all_host_bytes();
-                total += (b + 255) / 256 * 256;
-            }
-            if (!meas.empty()) {
-                char* host = pinned(total);
-                for (size_t i = 0; i < meas.size(); ++i) {
-                    if (kind[i] == 1) meas[i]->measure_begin(reinterpret_cast<double*>(host + off[i]));
-                    else meas[i]->measure_all_begin(host + off[i]);
-                }
-                if (hipStreamSynchronize(stream_->s) != hipSuccess) throw Error(ERR_HIP, "measurement launch failed");
-                for (size_t i = 0; i < meas.size(); ++i) {
-                    if (kind[i] == 1) meas[i]->measure_end(reinterpret_cast<const double*>(host + off[i]));
-                    else meas[i]->measure_all_end(host + off[i]);
-                }
-                stats_.sync_ops += meas.size();
-            }
-            lap(stats_.measure_s);
+            std::vector<Circuit*> still;
             for (size_t k = 0; k < active.size(); ++k) {
                 if (more[k]) {
-                    if (!measured[k]) {
-                        active[k]->run_next_op();
-                        stats_.sync_ops += 1;
-                    }
                     still.push_back(active[k]);
                 } else {
                     active[k]->finish();
                     if (!keep_states) active[k]->release_state();
                 }
             }
-            lap(stats_.other_s);
+            sync_ops(still);
+            tick = std::chrono::steady_clock::now();
             active.swap(still);
         }
     }

@@ -21,6 +21,10 @@ struct BatchStats {
     // where that time went: state set-up, gate queueing (matrices, analysis),
     // batched flushes (planning + kernels), batched measurements, other ops
     double start_s = 0.0, queue_s = 0.0, flush_s = 0.0, measure_s = 0.0, other_s = 0.0;
+    uint64_t column_launches = 0;   // kernels and copies of the batched column work
+    uint64_t solo_ops = 0;          // ops run circuit by circuit
+    double plan_s = 0.0;            // host planning inside the flushes
+    double pack_s = 0.0;            // host table building and upload inside the flushes
 };
 
 // Default consecutive tiles per workgroup step of a batched sweep (4096
@@ -75,13 +79,13 @@ private:
     std::vector<Circuit*> circuits_;
     std::shared_ptr<SharedStream> stream_;
     DeviceBuffer prog_;
+    PinnedBuffer stage_;
     BatchStats stats_;
-    char* host_ = nullptr;        // pinned: measurement scratch of a round
-    size_t host_cap_ = 0;
+    std::unique_ptr<ItemQueue> iq_;   // the column work of a round, all circuits
 
     size_t resident_bytes(const Circuit& c, size_t nr_shots) const;
     void run(const std::vector<Circuit*>& circuits, size_t nr_shots, bool keep_states);
-    char* pinned(size_t bytes);
+    void sync_ops(std::vector<Circuit*>& active);
 };
 
 }  // namespace q1t

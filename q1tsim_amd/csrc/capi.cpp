@@ -507,6 +507,10 @@ extern "C" int q1t_batch_stats_get(const q1t_batch* b, q1t_batch_stats* out)
     out->flush_s = s.flush_s;
     out->measure_s = s.measure_s;
     out->other_s = s.other_s;
+    out->column_launches = s.column_launches;
+    out->solo_ops = s.solo_ops;
+    out->plan_s = s.plan_s;
+    out->pack_s = s.pack_s;
     return 0;
 }
 

@@ -114,10 +114,11 @@ void Circuit::execute(size_t nr_shots)
     do_execute();
 }
 
-void Circuit::start(size_t nr_shots, std::shared_ptr<SharedStream> stream)
+void Circuit::start(size_t nr_shots, std::shared_ptr<SharedStream> stream, bool compact)
 {
     state_.reset();
-    state_ = std::make_unique<GpuVectorState>(nr_qbits_, nr_shots, prec_, std::move(stream));
+    state_ = std::make_unique<GpuVectorState>(nr_qbits_, nr_shots, prec_, std::move(stream),
+                                              compact && init_mode_ == 0);
     state_->set_fusion(fusion_k_);
     state_->set_tiling(tiling_);
     state_->set_graph(graph_);
@@ -191,20 +192,20 @@ bool Circuit::next_is_measure() const
     return op.type == OpType::Measure || op.type == OpType::Peek;
 }
 
-void Circuit::measure_begin(double* host)
+void Circuit::measure_enqueue(ItemQueue& q)
 {
     const CircuitOp& op = ops_[cursor_];
-    state_->marginal_w0_async(op.qbit, host, basis_code(op.basis));
+    q_item_ = state_->marginal_enqueue(q, op.qbit, basis_code(op.basis));
 }
 
 // The second half of measure_into (vectorstate.rs:264-275): a Binomial draw
 // per column as a count of runs with u < min(w0, 1), then the collapse.
-void Circuit::measure_end(const double* host)
+void Circuit::measure_complete(ItemQueue& q)
 {
     const CircuitOp& op = ops_[cursor_];
     GpuVectorState& st = *state_;
     if (c_state_.size() < st.nr_shots()) throw err_not_enough_space(c_state_.size(), st.nr_shots());
-    std::vector<double> w0 = st.marginal_w0_collect(op.qbit, host, basis_code(op.basis));
+    std::vector<double> w0 = st.marginal_collect(q, q_item_, op.qbit, basis_code(op.basis));
     RngSource rng = rng_.source();
     const std::vector<size_t>& counts = st.counts();
     std::vector<size_t> n0s(counts.size());
@@ -215,8 +216,9 @@ void Circuit::measure_end(const double* host)
             if (rng.uniform() < p) ++n0;
         n0s[c] = n0;
     }
-    st.measure_with_n0(op.qbit, op.cbit, n0s.data(), w0, c_state_.data(), c_state_.size(),
-                       op.type == OpType::Measure, basis_code(op.basis));
+    st.collapse_enqueue(q, op.qbit, op.cbit, n0s.data(), w0, c_state_.data(), c_state_.size(),
+                        op.type == OpType::Measure, basis_code(op.basis));
+    q_item_ = SIZE_MAX;
     ++cursor_;
 }
 
@@ -229,24 +231,56 @@ bool Circuit::next_is_z_measure_all() const
 
 // measure_all_into (vectorstate.rs:340-344): nr_shots uniforms from this
 // circuit's stream, then the sampling
-void Circuit::measure_all_begin(void* host)
+void Circuit::measure_all_enqueue(ItemQueue& q)
 {
     GpuVectorState& st = *state_;
     const CircuitOp& op = ops_[cursor_];
     if (c_state_.size() < st.nr_shots()) throw err_not_enough_space(c_state_.size(), st.nr_shots());
     if (op.cbits.size() != st.nr_bits()) throw err_invalid_nr_measurement_bits(op.cbits.size(), st.nr_bits());
     RngSource rng = rng_.source();
-    double* u = static_cast<double*>(host);
-    for (size_t r = 0; r < st.nr_shots(); ++r) u[r] = rng.uniform();
-    st.measure_all_async(host);
+    std::vector<double> u(st.nr_shots());
+    for (size_t r = 0; r < u.size(); ++r) u[r] = rng.uniform();
+    q_item_ = st.sample_enqueue(q, u.data());
 }
 
-void Circuit::measure_all_end(const void* host)
+void Circuit::measure_all_complete(const ItemQueue& q)
 {
     const CircuitOp& op = ops_[cursor_];
-    state_->measure_all_finish(op.cbits.data(), op.cbits.size(), host, c_state_.data(), c_state_.size(),
-                               op.type == OpType::MeasureAll);
+    state_->sample_finish(q, q_item_, op.cbits.data(), op.cbits.size(), c_state_.data(), c_state_.size(),
+                          op.type == OpType::MeasureAll, 0);
+    q_item_ = SIZE_MAX;
     ++cursor_;
+}
+
+bool Circuit::next_is_conditional() const
+{
+    return cursor_ < ops_.size() && ops_[cursor_].type == OpType::CondGate;
+}
+
+std::vector<uint8_t> Circuit::condition(const CircuitOp& op) const
+{
+    // control word: bit i of the word is c_state bit control[i] (circuit.rs:655-665)
+    std::vector<uint8_t> apply(c_state_.size());
+    for (size_t r = 0; r < c_state_.size(); ++r) {
+        uint64_t w = 0;
+        for (size_t i = 0; i < op.control.size(); ++i) w |= ((c_state_[r] >> op.control[i]) & 1) << i;
+        apply[r] = w == op.target;
+    }
+    return apply;
+}
+
+bool Circuit::conditional_enqueue(ItemQueue& q)
+{
+    const CircuitOp& op = ops_[cursor_];
+    const std::vector<uint8_t> apply = condition(op);
+    std::vector<double> pv;
+    for (const Param& p : op.params) pv.push_back(p.get());
+    std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());
+    if (!state_->conditional_enqueue(q, apply.data(), apply.size(), m.data(), gate_nbits(*op.gate), op.bits.data(),
+                                     op.bits.size(), op.gate->desc))
+        return false;
+    ++cursor_;
+    return true;
 }
 
 void Circuit::finish()
@@ -269,13 +303,7 @@ void Circuit::run_op(const CircuitOp& op)
         break;
     }
     case OpType::CondGate: {
-        // control word: bit i of the word is c_state bit control[i] (circuit.rs:655-665)
-        std::vector<uint8_t> apply(c_state_.size());
-        for (size_t r = 0; r < c_state_.size(); ++r) {
-            uint64_t w = 0;
-            for (size_t i = 0; i < op.control.size(); ++i) w |= ((c_state_[r] >> op.control[i]) & 1) << i;
-            apply[r] = w == op.target;
-        }
+        const std::vector<uint8_t> apply = condition(op);
         pv.clear();
         for (const Param& p : op.params) pv.push_back(p.get());
         std::vector<cdouble> m = gate_matrix(*op.gate, pv.data());

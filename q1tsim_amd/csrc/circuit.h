@@ -53,30 +53,39 @@ public:
     void reexecute();
 
     // Resumable execution (batches): `start` makes the fresh state that
-    // `execute` makes, on `stream`; `queue_gates` queues the gate ops from the
-    // cursor on, up to the next op that needs their result; `run_next_op`
-    // applies that op; `finish` flushes and keeps the stats.  `release_state`
-    // drops the state (the classical state stays).
-    void start(size_t nr_shots, std::shared_ptr<SharedStream> stream);
+    // `execute` makes, on `stream` (`compact`: |0..0> kept as a compact
+    // basis column until gates need it); `queue_gates` queues the gate ops
+    // from the cursor on, up to the next op that needs their result;
+    // `run_next_op` applies that op; `finish` flushes and keeps the stats.
+    // `release_state` drops the state (the classical state stays).
+    void start(size_t nr_shots, std::shared_ptr<SharedStream> stream, bool compact = false);
     bool queue_gates();
     void run_next_op();
     void finish();
     void release_state() { state_.reset(); }
-    // The op at the cursor is a measure or peek of one qubit (any basis): it
-    // can run in two steps, the marginal of many circuits' states enqueued
-    // before one synchronisation (batches).  measure_end completes it and
-    // advances.
-    bool next_is_measure() const;
+    // The op at the cursor as items of a queue that many circuits share
+    // (batches).  A measure or peek of one qubit (any basis) runs in two
+    // steps around the queue's launch and a synchronisation:
+    // measure_enqueue queues the marginals, measure_complete draws this
+    // circuit's outcomes, queues the collapses and advances.  A Z-basis
+    // measure_all / peek_all likewise (the uniforms drawn when queued).  A
+    // conditional gate is queued and advances in one step, unless it needs
+    // the per-gate kernels (then nothing is done and false returned).
     static int basis_code(char basis);   // 'Z' / 'X' / 'Y' -> 0 / 1 / 2
-    size_t measure_partials() const { return state_->marginal_partials(); }
-    void measure_begin(double* host);
-    void measure_end(const double* host);
-    // Same for a Z-basis measure_all / peek_all: begin draws this circuit's
-    // uniforms into `host` (measure_all_host_bytes) and enqueues the sampling.
+    bool at_end() const { return cursor_ >= ops_.size(); }
+    // a gate or barrier: left to the next queue_gates
+    bool next_is_gate() const
+    {
+        return cursor_ < ops_.size() && (ops_[cursor_].type == OpType::Gate || ops_[cursor_].type == OpType::Barrier);
+    }
+    bool next_is_measure() const;
+    void measure_enqueue(ItemQueue& q);
+    void measure_complete(ItemQueue& q);
     bool next_is_z_measure_all() const;
-    size_t measure_all_host_bytes() const { return state_->measure_all_host_bytes(); }
-    void measure_all_begin(void* host);
-    void measure_all_end(const void* host);
+    void measure_all_enqueue(ItemQueue& q);
+    void measure_all_complete(const ItemQueue& q);
+    bool next_is_conditional() const;
+    bool conditional_enqueue(ItemQueue& q);
     size_t nr_ops() const { return ops_.size(); }
     const std::vector<CircuitOp>& ops() const { return ops_; }
     Precision precision() const { return prec_; }
@@ -125,9 +134,11 @@ private:
     uint64_t init_seed_ = 0;
     RunStats stats_;
     size_t cursor_ = 0;
+    size_t q_item_ = SIZE_MAX;     // queue item / job of the op at the cursor
 
     void do_execute();
     void run_op(const CircuitOp& op);
+    std::vector<uint8_t> condition(const CircuitOp& op) const;
     void apply_named(GpuVectorState& st, const char* name, size_t q);
     void apply_named_all(GpuVectorState& st, const char* name);
 };

@@ -4,11 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 
 #include "common.h"
 #include "gates.h"
+#include "items.h"
 
 namespace q1t {
 
@@ -69,18 +72,23 @@ struct PendingGate {
     std::vector<int> qbits;       // index-bit positions of the listed qubits
 };
 
-// Process-wide cache of freed device blocks, reused by exact size (states of
-// a batch come and go with the same few sizes).  A released block carries an
-// event recorded on its stream: the same stream reuses it at once (stream
-// order), another stream only once the event has completed.  Cached bytes
-// are capped (default: half of device memory; a batch's states come and go
-// by the hundreds of GB): a release over the cap frees the oldest cached
-// blocks first.  On allocation failure the cache is emptied and the
-// allocation retried.
+// Process-wide cache of freed device blocks.  Requests are rounded up to a
+// size class -- a power of two up to 1 GiB, a multiple of 256 MiB above --
+// and blocks are reused by class (the states of a batch come and go with
+// many shapes, 2^n amplitudes times a few columns, but few classes).  A
+// released block carries an event recorded on its stream: the same stream
+// reuses it at once (stream order), another stream only once the event has
+// completed.  Cached bytes are capped (default: 7/8 of device memory; a
+// batch's states come and go by the hundreds of GB): a release over the cap
+// frees the oldest cached blocks first.  On allocation failure the cache is
+// emptied and the allocation retried.
 class DeviceCache {
 public:
     static DeviceCache& get();
+    // bytes a request of `bytes` takes
+    static size_t size_class(size_t bytes);
     void* alloc(size_t bytes, hipStream_t s);
+    // `bytes` as passed to alloc
     void release(void* p, size_t bytes, hipStream_t s);
     void trim();
     // `s` has been synchronised and is about to be destroyed
@@ -94,11 +102,13 @@ private:
     void evict_to(size_t limit);   // caller holds mu_
     struct Entry {
         void* p;
-        size_t bytes;
+        size_t bytes;     // the size class
         hipStream_t s;    // null with ev: idle, reusable on any stream
         hipEvent_t ev;
+        uint64_t seq;     // release order
     };
-    std::vector<Entry> free_;
+    std::map<size_t, std::deque<Entry>> free_;   // by size class, oldest first
+    uint64_t seq_ = 0;
     size_t cached_ = 0, cap_ = 0;
     std::mutex mu_;
 };
@@ -119,8 +129,10 @@ double measure_copy_ms(size_t bytes, int reps);
 class GpuVectorState {
 public:
     // `shared`: run on a stream shared with other states (batches) instead
-    // of a stream of its own.
-    GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared = nullptr);
+    // of a stream of its own; `compact_zero`: start as the compact basis
+    // column |0..0> (set_zero_state_compact).
+    GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared = nullptr,
+                   bool compact_zero = false);
     ~GpuVectorState();
     GpuVectorState(const GpuVectorState&) = delete;
     GpuVectorState& operator=(const GpuVectorState&) = delete;
@@ -152,12 +164,6 @@ public:
     // the device in the host's order (bit-identical results).  Needs no
     // pending gates.  Calls that overlap on one GPU race on that array.
     void marginal_w0_noalloc(size_t qbit, double* w0s, size_t len);
-    // marginal_w0 in two steps (batches): enqueue the reduction and its copy
-    // into `host` (pinned, >= marginal_partials() doubles); once the stream
-    // has synchronised, collect gives exactly what marginal_w0 gives.
-    size_t marginal_partials() const;
-    void marginal_w0_async(size_t qbit, double* host, int basis = 0);
-    std::vector<double> marginal_w0_collect(size_t qbit, const double* host, int basis = 0) const;
     void measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
                          uint64_t* res, size_t res_len, bool collapse, int basis = 0);
     void measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng, bool collapse,
@@ -170,25 +176,46 @@ public:
                               size_t res_len, bool collapse, int rot = 0);
     void measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len, RngSource& rng,
                           bool collapse, int rot = 0);
-    // measure_all in two steps (batches): `host` (pinned, measure_all_host_bytes)
-    // starts with the nr_shots uniforms; async enqueues the sampling of every
-    // column and the copies back; once the stream has synchronised, finish
-    // gives exactly what measure_all_uniforms gives.
-    size_t measure_all_host_bytes() const;
-    void measure_all_async(void* host);
-    void measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res, size_t res_len,
-                            bool collapse, int rot = 0);
     void reset(size_t bit, RngSource& rng);
     void reset_all();
+
+    // Column work as items of an ItemQueue (batches queue many states' work
+    // and launch it together; the operations above run a queue of their own
+    // with the same kernels).  After the queue's launch() and a stream
+    // synchronisation the results are read back.
+    //  - marginal: the outcome-0 weight of qbit per column; returns the first
+    //    item (one per column, in order), SIZE_MAX when the state knows the
+    //    weights without the device (compact basis columns)
+    size_t marginal_enqueue(ItemQueue& q, size_t qbit, int basis);
+    std::vector<double> marginal_collect(const ItemQueue& q, size_t first, size_t qbit, int basis) const;
+    //  - measure_with_n0 with its collapses (and the release of replaced
+    //    columns) queued
+    void collapse_enqueue(ItemQueue& q, size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
+                          uint64_t* res, size_t res_len, bool collapse, int basis);
+    //  - measure_all: the tree levels and walks of nr_shots runs with
+    //    uniforms u; SIZE_MAX for compact basis columns (nothing queued)
+    size_t sample_enqueue(ItemQueue& q, const double* u);
+    void sample_finish(const ItemQueue& q, size_t job, const size_t* cbits, size_t ncbits, uint64_t* res,
+                       size_t res_len, bool collapse, int rot);
+    //  - apply_conditional_gate; false (nothing done) when the gate needs the
+    //    per-gate kernels or the column set changes
+    bool conditional_enqueue(ItemQueue& q, const uint8_t* control, size_t ncontrol, const cdouble* mat,
+                             size_t nr_affected, const size_t* qbits, size_t nbits, const std::string& desc);
+    //  - compact basis columns written out as dense columns
+    void materialize_enqueue(ItemQueue& q);
+    // the state starts as the compact basis column |0..0> (no device memory
+    // until gates need it)
+    void set_zero_state_compact();
 
     // build-added
     void read_amplitudes(size_t col, cdouble* out);
     void flush();
     void synchronize();
     // batched flushes (many states, one launch per sweep round): the queue is
-    // taken over by the batch runner, which applies it and reports stats here
+    // taken over by the batch runner, which applies it and reports stats
+    // here; compact columns are written out through `q` first
     std::vector<PendingGate>& pending() { return queue_; }
-    void prepare_batched_flush();
+    void prepare_batched_flush(ItemQueue& q);
     void finish_batched_flush(const RunStats& s);
     const RunStats& last_stats() const { return stats_; }
     void set_fusion(int max_block_bits) { fusion_k_ = max_block_bits; }
@@ -312,6 +339,8 @@ private:
     size_t small_cap_ = 0;
     void* d_slot_[SLOT_COUNT] = {nullptr, nullptr, nullptr};
     size_t slot_cap_[SLOT_COUNT] = {0, 0, 0};
+    std::unique_ptr<ItemQueue> iq_;       // the single-state operations' queue
+    ItemQueue& iq();
 
     CostModel* cost_model_ = nullptr;
     int plan_mode_ = PLAN_GREEDY;
@@ -343,9 +372,15 @@ private:
     uint64_t dim() const { return uint64_t(1) << nr_bits_; }
     void check_qbit(size_t q) const;
     void alloc_cols(size_t ncols);
-    void launch_marginal(size_t qbit, double* part, int basis = 0);
-    std::vector<double> w0_from_partials(size_t qbit, const double* h, int basis = 0) const;
+    std::vector<double> basis_w0(size_t qbit, int basis) const;
     static void check_basis(int basis);
+    struct CondRange {
+        size_t col, len;
+        bool apply;
+    };
+    std::vector<CondRange> conditional_ranges(const uint8_t* control) const;
+    void check_conditional(size_t ncontrol, size_t nr_affected, const size_t* qbits, size_t nbits,
+                           const std::string& desc) const;
     void* dev_alloc(size_t bytes);
     void dev_release(void* p, size_t bytes);
     void* small_buf(size_t bytes);

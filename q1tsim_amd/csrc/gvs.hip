@@ -226,234 +226,6 @@ __global__ void __launch_bounds__(kBlock) k_norm_partial(const V* __restrict__ a
 // of outcome 1 b1 = (a0 - a1)/sqrt2 or (a0 + i a1)/sqrt2.
 enum MeasBasis : int { BASIS_Z = 0, BASIS_X = 1, BASIS_Y = 2 };
 
-// Partial sums of |b0|^2 over the pairs of bit p (FP64), per column.
-template <typename V, int B>
-__global__ void __launch_bounds__(kBlock) k_norm_partial_basis(const V* __restrict__ amp, uint64_t col_stride,
-                                                               uint64_t nwork, int p, double* __restrict__ partial)
-{
-    const V* a = amp + uint64_t(blockIdx.y) * col_stride;
-    const uint64_t bit = uint64_t(1) << p;
-    double acc = 0.0;
-    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
-        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1));
-        const V x0 = a[i0], x1 = a[i0 | bit];
-        double re, im;
-        if constexpr (B == BASIS_X) {
-            re = double(x0.x) + double(x1.x);
-            im = double(x0.y) + double(x1.y);
-        } else {   // a0 - i a1
-            re = double(x0.x) + double(x1.y);
-            im = double(x0.y) - double(x1.x);
-        }
-        acc += 0.5 * __fma_rn(re, re, im * im);
-    }
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) partial[uint64_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
-}
-
-// Collapse onto outcome 0 (dst0, scale s0 = 1/sqrt(w0)) and/or 1 (dst1,
-// s1 = 1/sqrt(1 - w0)) of an X- or Y-basis measurement, the basis change back
-// included: X, outcome 0: a0' = a1' = (a0 + a1) s0/2; outcome 1:
-// a0' = -a1' = (a0 - a1) s1/2.  Y, outcome 0: u = a0 - i a1, a0' = u s0/2,
-// a1' = i u s0/2; outcome 1: v = a0 + i a1, a0' = v s1/2, a1' = -i v s1/2.
-template <typename V, int B>
-__global__ void __launch_bounds__(kBlock) k_collapse_basis(const V* src, V* dst0, V* dst1, uint64_t nwork, int p,
-                                                           typename RealOf<V>::type s0,
-                                                           typename RealOf<V>::type s1)
-{
-    using Rl = typename RealOf<V>::type;
-    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
-    const uint64_t bit = uint64_t(1) << p;
-    const Rl h0 = Rl(0.5) * s0, h1 = Rl(0.5) * s1;
-    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
-        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1)), i1 = i0 | bit;
-        const V x0 = src[i0], x1 = src[i1];
-        if constexpr (B == BASIS_X) {
-            const V u = mkc<V>((x0.x + x1.x) * h0, (x0.y + x1.y) * h0);
-            const V v = mkc<V>((x0.x - x1.x) * h1, (x0.y - x1.y) * h1);
-            if (dst0) {
-                dst0[i0] = u;
-                dst0[i1] = u;
-            }
-            if (dst1) {
-                dst1[i0] = v;
-                dst1[i1] = mkc<V>(-v.x, -v.y);
-            }
-        } else {
-            const V u = mkc<V>((x0.x + x1.y) * h0, (x0.y - x1.x) * h0);   // (a0 - i a1) s0/2
-            const V v = mkc<V>((x0.x - x1.y) * h1, (x0.y + x1.x) * h1);   // (a0 + i a1) s1/2
-            if (dst0) {
-                dst0[i0] = u;
-                dst0[i1] = mkc<V>(-u.y, u.x);   // i u
-            }
-            if (dst1) {
-                dst1[i0] = v;
-                dst1[i1] = mkc<V>(v.y, -v.x);   // -i v
-            }
-        }
-    }
-}
-
-// Dense columns of compact basis columns: column y holds |k> with k =
-// ks[y] (rot 0), H^n|k> (rot 1, after an X-basis measure_all) or
-// S^n H^n|k> (rot 2, Y basis): amplitude j = (-1)^popcount(j&k) i^popcount(j)
-// (the i factor for rot 2 only) times `mag` = 2^(-n/2); rot 0 writes the one
-// amplitude 1 and zeros.
-template <typename V>
-__global__ void __launch_bounds__(kBlock) k_basis_columns(V* __restrict__ amp, uint64_t col_stride, uint64_t dim,
-                                                          const uint64_t* __restrict__ ks, int rot,
-                                                          typename RealOf<V>::type mag)
-{
-    using Rl = typename RealOf<V>::type;
-    V* a = amp + uint64_t(blockIdx.y) * col_stride;
-    const uint64_t k = ks[blockIdx.y];
-    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < dim; j += step) {
-        V v;
-        if (rot == 0) {
-            v = mkc<V>(j == k ? Rl(1) : Rl(0), Rl(0));
-        } else {
-            const Rl s = (__popcll(j & k) & 1) ? -mag : mag;
-            const int q = rot == 2 ? (__popcll(j) & 3) : 0;   // i^q
-            v = q == 0 ? mkc<V>(s, 0) : q == 1 ? mkc<V>(0, s) : q == 2 ? mkc<V>(-s, 0) : mkc<V>(0, -s);
-        }
-        a[j] = v;
-    }
-}
-
-// Level sums of the canonical sampling tree: out[g] = left fold of
-// in[16g .. 16g+15].  A workgroup folds 4096 inputs; loads are coalesced
-// into LDS in a transposed, padded layout so each thread reads its 16
-// values bank-conflict free.
-template <typename T> __device__ __forceinline__ double leaf_value(const T* in, uint64_t k);
-template <> __device__ __forceinline__ double leaf_value<double2>(const double2* in, uint64_t k) { return weight(in[k]); }
-template <> __device__ __forceinline__ double leaf_value<float2>(const float2* in, uint64_t k) { return weight(in[k]); }
-template <> __device__ __forceinline__ double leaf_value<double>(const double* in, uint64_t k) { return in[k]; }
-
-// One tree level of every column at once: column blockIdx.y reads at
-// in + y * in_stride and writes at out + y * out_stride.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_tree_level(const T* __restrict__ in, uint64_t in_stride, uint64_t size,
-                                                       double* __restrict__ out, uint64_t out_stride)
-{
-    in += uint64_t(blockIdx.y) * in_stride;
-    out += uint64_t(blockIdx.y) * out_stride;
-    __shared__ double w[kTreeFanout * (kBlock + 1)];
-    const uint64_t nout = (size + kTreeFanout - 1) / kTreeFanout;
-    const uint64_t chunk = uint64_t(kBlock) * kTreeFanout;
-    for (uint64_t base = uint64_t(blockIdx.x) * chunk; base < size; base += uint64_t(gridDim.x) * chunk) {
-#pragma unroll
-        for (int r = 0; r < kTreeFanout; ++r) {
-            const int e = r * kBlock + threadIdx.x;
-            const uint64_t k = base + e;
-            const double v = k < size ? leaf_value<T>(in, k) : 0.0;
-            w[(e & (kTreeFanout - 1)) * (kBlock + 1) + (e >> 4)] = v;
-        }
-        __syncthreads();
-        const uint64_t g = base / kTreeFanout + threadIdx.x;
-        if (g < nout) {
-            double acc = w[threadIdx.x];
-            for (int jj = 1; jj < kTreeFanout; ++jj)
-                if (g * kTreeFanout + jj < size) acc += w[jj * (kBlock + 1) + threadIdx.x];
-            out[g] = acc;
-        }
-        __syncthreads();
-    }
-}
-
-struct TreeDesc {
-    int nlev;
-    uint64_t size[16];
-    const double* lev[16];   // lev[0] unused: level 0 is read from the amplitudes
-};
-
-// Entry k of level l of the column whose tree levels start `coff` entries
-// after column 0's (level 0 is that column's amplitudes `a`).
-template <typename V>
-__device__ __forceinline__ double tree_value(const V* a, const TreeDesc& td, int l, uint64_t k, uint64_t coff)
-{
-    return l == 0 ? weight(a[k]) : td.lev[l][coff + k];
-}
-
-// Inverse-CDF sampling of one run per thread by the canonical tree walk:
-// t = u * T; within each group keep offset += S_j while offset + S_j <= t,
-// descend into the first child where that fails, or the last positive one.
-// All runs of all columns in one launch: run r belongs to the last column c
-// with first_run[c] <= r (columns in order; first_run[0] = 0).
-template <typename V>
-__global__ void __launch_bounds__(kBlock) k_tree_sample(const V* __restrict__ a0, uint64_t col_stride, TreeDesc td,
-                                                        uint64_t per_col, const uint64_t* __restrict__ first_run,
-                                                        int ncols, const double* __restrict__ u01, uint64_t nruns,
-                                                        uint64_t* __restrict__ out)
-{
-    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= nruns) return;
-    int lo = 0, hi = ncols - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (first_run[mid] <= r) lo = mid;
-        else hi = mid - 1;
-    }
-    const V* a = a0 + uint64_t(lo) * col_stride;
-    const uint64_t coff = uint64_t(lo) * per_col;
-    const int top = td.nlev - 1;
-    double total = tree_value(a, td, top, 0, coff);
-    for (uint64_t k = 1; k < td.size[top]; ++k) total += tree_value(a, td, top, k, coff);
-    const double t = u01[r] * total;
-    double offset = 0.0;
-    uint64_t gstart = 0, glen = td.size[top], idx = 0;
-    for (int l = top; l >= 0; --l) {
-        double v[kTreeFanout];
-        int last = 0;
-#pragma unroll
-        for (int j = 0; j < kTreeFanout; ++j) {
-            v[j] = (uint64_t(j) < glen) ? tree_value(a, td, l, gstart + j, coff) : 0.0;
-            if (v[j] > 0.0) last = j;
-        }
-        int chosen = last;
-        for (int j = 0; j < last; ++j) {
-            const double nx = offset + v[j];
-            if (nx <= t) {
-                offset = nx;
-            } else {
-                chosen = j;
-                break;
-            }
-        }
-        idx = gstart + uint64_t(chosen);
-        if (l == 0) break;
-        gstart = idx * kTreeFanout;
-        const uint64_t rem = td.size[l - 1] - gstart;
-        glen = rem < uint64_t(kTreeFanout) ? rem : uint64_t(kTreeFanout);
-    }
-    out[r] = idx;
-}
-
-// Fused collapse + renormalise of one column onto qubit bit p:
-// dst0 keeps the bit-0 half scaled by s0, dst1 the bit-1 half scaled by s1
-// (either may be null; src may alias either destination).
-template <typename V>
-__global__ void __launch_bounds__(kBlock) k_collapse(const V* src, V* dst0, V* dst1, uint64_t nwork, int p,
-                                                     typename RealOf<V>::type s0, typename RealOf<V>::type s1)
-{
-    const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
-    const uint64_t bit = uint64_t(1) << p;
-    const V zero = mkc<V>(0, 0);
-    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < nwork; j += step) {
-        const uint64_t i0 = ((j >> p) << (p + 1)) | (j & (bit - 1)), i1 = i0 | bit;
-        const V x0 = src[i0], x1 = src[i1];
-        if (dst0) {
-            dst0[i0] = mkc<V>(x0.x * s0, x0.y * s0);
-            dst0[i1] = zero;
-        }
-        if (dst1) {
-            dst1[i0] = zero;
-            dst1[i1] = mkc<V>(x1.x * s1, x1.y * s1);
-        }
-    }
-}
-
 // Product state: amplitude = b_{n-1}[x_{n-1}] * (... * (b_0[x_0] * 1)), the
 // association order of successive kron_mat calls (vectorstate.rs:68-74).
 template <typename V>
@@ -547,16 +319,33 @@ DeviceCache& DeviceCache::get()
     return *c;
 }
 
+size_t DeviceCache::size_class(size_t bytes)
+{
+    if (bytes <= 4096) return 4096;
+    if (bytes <= (size_t(1) << 30)) {
+        size_t c = 4096;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t g = size_t(256) << 20;
+    return (bytes + g - 1) / g * g;
+}
+
+static void drop_entry_event(hipEvent_t ev, bool wait)
+{
+    if (!ev) return;
+    if (wait) HIP_OK(hipEventSynchronize(ev));
+    HIP_OK(hipEventDestroy(ev));
+}
+
 void DeviceCache::trim()
 {
     std::lock_guard<std::mutex> lk(mu_);
-    for (Entry& e : free_) {
-        if (e.ev) {
-            HIP_OK(hipEventSynchronize(e.ev));
-            HIP_OK(hipEventDestroy(e.ev));
+    for (auto& kv : free_)
+        for (Entry& e : kv.second) {
+            drop_entry_event(e.ev, true);
+            HIP_OK(hipFree(e.p));
         }
-        HIP_OK(hipFree(e.p));
-    }
     free_.clear();
     cached_ = 0;
 }
@@ -565,20 +354,24 @@ size_t DeviceCache::default_cap()
 {
     size_t f = 0, t = 0;
     HIP_OK(hipMemGetInfo(&f, &t));
-    return t / 2;
+    return t / 8 * 7;
 }
 
 void DeviceCache::evict_to(size_t limit)
 {
-    while (cached_ > limit && !free_.empty()) {
-        Entry& e = free_.front();
-        if (e.ev) {
-            HIP_OK(hipEventSynchronize(e.ev));
-            HIP_OK(hipEventDestroy(e.ev));
-        }
+    while (cached_ > limit) {
+        // the oldest block over all classes
+        auto oldest = free_.end();
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (!it->second.empty() && (oldest == free_.end() || it->second.front().seq < oldest->second.front().seq))
+                oldest = it;
+        if (oldest == free_.end()) break;
+        Entry e = oldest->second.front();
+        oldest->second.pop_front();
+        if (oldest->second.empty()) free_.erase(oldest);
+        drop_entry_event(e.ev, true);
         HIP_OK(hipFree(e.p));
         cached_ -= e.bytes;
-        free_.erase(free_.begin());
     }
 }
 
@@ -600,43 +393,49 @@ size_t DeviceCache::cached_bytes()
 void DeviceCache::forget_stream(hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(mu_);
-    for (Entry& e : free_) {
-        if (e.s != s) continue;
-        if (e.ev) (void)hipEventDestroy(e.ev);
-        e.ev = nullptr;
-        e.s = nullptr;
-    }
+    for (auto& kv : free_)
+        for (Entry& e : kv.second) {
+            if (e.s != s) continue;
+            if (e.ev) (void)hipEventDestroy(e.ev);
+            e.ev = nullptr;
+            e.s = nullptr;
+        }
 }
 
 void* DeviceCache::alloc(size_t bytes, hipStream_t s)
 {
+    const size_t cls = size_class(bytes);
     {
         std::lock_guard<std::mutex> lk(mu_);
-        for (size_t i = 0; i < free_.size(); ++i) {
-            Entry& e = free_[i];
-            if (e.bytes != bytes) continue;
-            if (e.ev && e.s != s && hipEventQuery(e.ev) != hipSuccess) {
-                (void)hipGetLastError();   // a pending event is not an error: keep it out of the next check
-                continue;
+        auto it = free_.find(cls);
+        if (it != free_.end()) {
+            std::deque<Entry>& v = it->second;
+            for (size_t i = 0; i < v.size(); ++i) {
+                Entry& e = v[i];
+                if (e.ev && e.s != s && hipEventQuery(e.ev) != hipSuccess) {
+                    (void)hipGetLastError();   // a pending event is not an error: keep it out of the next check
+                    continue;
+                }
+                void* p = e.p;
+                if (e.ev) HIP_OK(hipEventDestroy(e.ev));
+                cached_ -= e.bytes;
+                v.erase(v.begin() + std::ptrdiff_t(i));   // keeps release order (oldest first)
+                if (v.empty()) free_.erase(it);
+                return p;
             }
-            void* p = e.p;
-            if (e.ev) HIP_OK(hipEventDestroy(e.ev));
-            cached_ -= e.bytes;
-            free_.erase(free_.begin() + std::ptrdiff_t(i));   // keeps release order (oldest first)
-            return p;
         }
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, cls);
     if (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
         trim();
-        e = hipMalloc(&p, bytes);
+        e = hipMalloc(&p, cls);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw Error(e == hipErrorOutOfMemory ? ERR_OUT_OF_MEMORY : ERR_HIP,
-                    "Device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+                    "Device allocation of " + std::to_string(cls) + " bytes failed: " + hipGetErrorString(e));
     }
     return p;
 }
@@ -644,21 +443,22 @@ void* DeviceCache::alloc(size_t bytes, hipStream_t s)
 void DeviceCache::release(void* p, size_t bytes, hipStream_t s)
 {
     if (!p) return;
+    const size_t cls = size_class(bytes);
     std::lock_guard<std::mutex> lk(mu_);
     if (!cap_) cap_ = default_cap();
-    if (bytes > cap_) {
+    if (cls > cap_) {
         HIP_OK(hipStreamSynchronize(s));
         HIP_OK(hipFree(p));
         return;
     }
     // over the cap: free the oldest cached blocks, so the cache follows the
     // sizes in use now (a batch after a large state, c128 after c64)
-    evict_to(cap_ - bytes);
+    evict_to(cap_ - cls);
     hipEvent_t ev;
     HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     HIP_OK(hipEventRecord(ev, s));
-    free_.push_back(Entry{p, bytes, s, ev});
-    cached_ += bytes;
+    free_[cls].push_back(Entry{p, cls, s, ev, seq_++});
+    cached_ += cls;
 }
 
 SharedStream::SharedStream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -672,7 +472,8 @@ SharedStream::~SharedStream()
     }
 }
 
-GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared)
+GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, std::shared_ptr<SharedStream> shared,
+                               bool compact_zero)
     : nr_bits_(nr_bits), nr_shots_(nr_shots), prec_(prec)
 {
     if (nr_bits > 40) throw Error(ERR_INVALID_ARGUMENT, "Too many qubits for a dense state");
@@ -687,12 +488,22 @@ GpuVectorState::GpuVectorState(size_t nr_bits, size_t nr_shots, Precision prec, 
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     }
     geom_ = tile_geometry(prec);
-    set_zero_state();
+    if (compact_zero) set_zero_state_compact();
+    else set_zero_state();
 }
 
-void GpuVectorState::prepare_batched_flush()
+void GpuVectorState::set_zero_state_compact()
 {
-    materialize();
+    queue_.clear();
+    basis_ = true;
+    basis_rot_ = 0;
+    basis_idx_.assign(1, 0);
+    counts_.assign(1, nr_shots_);
+}
+
+void GpuVectorState::prepare_batched_flush(ItemQueue& q)
+{
+    materialize_enqueue(q);
     stats_ = RunStats{};
 }
 
@@ -724,6 +535,10 @@ GpuVectorState::~GpuVectorState()
     // an own stream is destroyed below, so its work must end first; a shared
     // stream outlives the state, and the released blocks are event-guarded
     if (stream_ && !shared_) (void)hipStreamSynchronize(stream_);
+    try {
+        ItemQueuePool::give(std::move(iq_));   // synchronises the queue's last stream
+    } catch (...) {
+    }
     try {
         DeviceCache& dc = DeviceCache::get();
         for (FlushGraph& fg : fgraphs_) {
@@ -900,37 +715,42 @@ void GpuVectorState::set_random_normal(uint64_t seed)
     HIP_OK(hipStreamSynchronize(stream_));
 }
 
+ItemQueue& GpuVectorState::iq()
+{
+    if (!iq_) iq_ = ItemQueuePool::take(prec_ == PREC_C64);
+    return *iq_;
+}
+
 void GpuVectorState::materialize()
+{
+    if (!basis_) return;
+    ItemQueue& q = iq();
+    materialize_enqueue(q);
+    q.launch(stream_);
+}
+
+void GpuVectorState::materialize_enqueue(ItemQueue& q)
 {
     if (!basis_) return;
     const size_t nc = counts_.size();
     const size_t bytes_needed = nc * dim() * elem_bytes();
-    size_t free_b = 0, total_b = 0;
-    HIP_OK(hipMemGetInfo(&free_b, &total_b));
-    // blocks idle in the device cache are freed by its allocation retry, so they count as free
-    size_t have = amp_ ? amp_cols_ * dim() * elem_bytes() : 0;
-    if (bytes_needed > free_b + have + DeviceCache::get().cached_bytes())
-        throw Error(ERR_OUT_OF_MEMORY, "Not enough device memory to expand " + std::to_string(nc) +
-                                           " measured basis columns into dense columns");
+    const size_t have = amp_ ? amp_cols_ * dim() * elem_bytes() : 0;
+    if (bytes_needed > have && bytes_needed >= (size_t(1) << 28)) {
+        // a large expansion: a clear error before the allocation is tried
+        // (blocks idle in the device cache are freed by its allocation
+        // retry, so they count as free)
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        if (bytes_needed > free_b + have + DeviceCache::get().cached_bytes())
+            throw Error(ERR_OUT_OF_MEMORY, "Not enough device memory to expand " + std::to_string(nc) +
+                                               " measured basis columns into dense columns");
+    }
     alloc_cols(nc);
-    // the basis indices, then one launch writes every column (kernel writes
-    // only: no memset node if a caller captures it)
-    uint64_t* ks = static_cast<uint64_t*>(small_buf(nc * sizeof(uint64_t)));
-    HIP_OK(hipMemcpyAsync(ks, basis_idx_.data(), nc * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+    // every amplitude written by the kernel (no memset node if a caller
+    // captures the launch)
     const double mag = (nr_bits_ & 1) ? std::ldexp(0.70710678118654752440, -int(nr_bits_ / 2))
                                       : std::ldexp(1.0, -int(nr_bits_ / 2));
-    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
-        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
-        const dim3 g(grid_for(dim()), ny);
-        if (prec_ == PREC_C128)
-            hipLaunchKernelGGL(k_basis_columns<double2>, g, dim3(kBlock), 0, stream_,
-                               static_cast<double2*>(col_ptr(c0)), dim(), dim(), ks + c0, basis_rot_, mag);
-        else
-            hipLaunchKernelGGL(k_basis_columns<float2>, g, dim3(kBlock), 0, stream_,
-                               static_cast<float2*>(col_ptr(c0)), dim(), dim(), ks + c0, basis_rot_, float(mag));
-        HIP_OK(hipGetLastError());
-    }
-    HIP_OK(hipStreamSynchronize(stream_));
+    for (size_t c = 0; c < nc; ++c) q.add_basis(BasisItem{col_ptr(c), dim(), basis_idx_[c], 0, mag, basis_rot_, 0});
     basis_ = false;
     basis_rot_ = 0;
     basis_idx_.clear();
@@ -1111,20 +931,11 @@ void GpuVectorState::apply_op(const GateOp& op, size_t cb, size_t ce)
         launch_op<float2>(op, static_cast<float2*>(col_ptr(cb)), dim(), ce - cb, int(nr_bits_), stream_, *this, stats_);
 }
 
-void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
-                                            size_t nr_affected, const size_t* qbits, size_t nbits,
-                                            const std::string& desc)
+// collect_conditional_ranges (qustate.rs:100-127): runs of equal control
+// within each column
+std::vector<GpuVectorState::CondRange> GpuVectorState::conditional_ranges(const uint8_t* control) const
 {
-    if (ncontrol != nr_shots_) throw err_invalid_nr_control_bits(ncontrol, nr_shots_, desc);
-    if (nr_affected != nbits) throw err_invalid_nr_bits(nbits, nr_affected, desc);
-    for (size_t k = 0; k < nbits; ++k) check_qbit(qbits[k]);
-    flush();
-    // collect_conditional_ranges (qustate.rs:100-127)
-    struct Range {
-        size_t col, len;
-        bool apply;
-    };
-    std::vector<Range> ranges;
+    std::vector<CondRange> ranges;
     size_t off = 0;
     for (size_t icol = 0; icol < counts_.size(); ++icol) {
         const size_t count = counts_[icol];
@@ -1142,8 +953,60 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
         if (begin < off + count) ranges.push_back({icol, off + count - begin, prev});
         off += count;
     }
+    return ranges;
+}
+
+void GpuVectorState::check_conditional(size_t ncontrol, size_t nr_affected, const size_t* qbits, size_t nbits,
+                                       const std::string& desc) const
+{
+    if (ncontrol != nr_shots_) throw err_invalid_nr_control_bits(ncontrol, nr_shots_, desc);
+    if (nr_affected != nbits) throw err_invalid_nr_bits(nbits, nr_affected, desc);
+    for (size_t k = 0; k < nbits; ++k) check_qbit(qbits[k]);
+}
+
+bool GpuVectorState::conditional_enqueue(ItemQueue& q, const uint8_t* control, size_t ncontrol, const cdouble* mat,
+                                         size_t nr_affected, const size_t* qbits, size_t nbits,
+                                         const std::string& desc)
+{
+    check_conditional(ncontrol, nr_affected, qbits, nbits, desc);
+    flush();
+    const std::vector<CondRange> ranges = conditional_ranges(control);
     bool any = false;
-    for (const Range& r : ranges) any = any || r.apply;
+    for (const CondRange& r : ranges) any = any || r.apply;
+    bool same_layout = ranges.size() == counts_.size();
+    for (size_t r = 0; same_layout && r < ranges.size(); ++r) same_layout = ranges[r].col == r;
+    // a column set that changes (runs of one column disagree) is copied
+    // column by column: left to apply_conditional_gate
+    if (!same_layout) return false;
+    if (basis_ && !any) return true;
+    GateOp op = analyze_gate(mat, nbits, qbits, nr_bits_);
+    ColOp co;
+    if (op.kind != OpKind::Identity && !col_op_from(op, co)) return false;
+    materialize_enqueue(q);
+    if (op.kind == OpKind::Identity || !any) return true;
+    const uint32_t oi = q.add_op(co);
+    const uint64_t nwork = dim() >> co.npos;
+    for (size_t r = 0; r < ranges.size(); ++r)
+        if (ranges[r].apply) q.add_op_item(col_ptr(r), nwork, oi);
+    return true;
+}
+
+void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncontrol, const cdouble* mat,
+                                            size_t nr_affected, const size_t* qbits, size_t nbits,
+                                            const std::string& desc)
+{
+    check_conditional(ncontrol, nr_affected, qbits, nbits, desc);
+    flush();
+    {
+        ItemQueue& q = iq();
+        if (conditional_enqueue(q, control, ncontrol, mat, nr_affected, qbits, nbits, desc)) {
+            q.launch(stream_);
+            return;
+        }
+    }
+    const std::vector<CondRange> ranges = conditional_ranges(control);
+    bool any = false;
+    for (const CondRange& r : ranges) any = any || r.apply;
     bool same_layout = ranges.size() == counts_.size();
     for (size_t r = 0; same_layout && r < ranges.size(); ++r) same_layout = ranges[r].col == r;
     if (basis_ && same_layout && !any) return;
@@ -1163,7 +1026,18 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
         amp_cols_ = nr;
     }
     counts_.clear();
-    for (const Range& r : ranges) counts_.push_back(r.len);
+    for (const CondRange& r : ranges) counts_.push_back(r.len);
+    // gates of the column kernels; anything wider by the per-gate kernels
+    ColOp co;
+    if (op.kind != OpKind::Identity && col_op_from(op, co)) {
+        ItemQueue& q = iq();
+        const uint32_t oi = q.add_op(co);
+        const uint64_t nwork = dim() >> co.npos;
+        for (size_t r = 0; r < ranges.size(); ++r)
+            if (ranges[r].apply) q.add_op_item(col_ptr(r), nwork, oi);
+        q.launch(stream_);
+        return;
+    }
     for (size_t r = 0; r < ranges.size();) {
         if (!ranges[r].apply) {
             ++r;
@@ -1178,106 +1052,55 @@ void GpuVectorState::apply_conditional_gate(const uint8_t* control, size_t ncont
 
 // ---------------------------------------------------------- measurement
 
-// Per-column partial sums of sum_{bit p = 0} |a|^2 (FP64), kNormBlocks per
-// column, into `part` (device) on the state's stream.
-static constexpr unsigned kNormBlocks = 256;
-
-void GpuVectorState::launch_marginal(size_t qbit, double* part, int basis)
-{
-    const size_t nc = counts_.size();
-    const int p = int(nr_bits_ - 1 - qbit);
-    const PosList pl = make_pos({p});
-    const uint64_t nwork = dim() / 2;
-    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
-        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
-        const dim3 grid(kNormBlocks, ny);
-        if (basis != BASIS_Z) {
-            if (prec_ == PREC_C128) {
-                const double2* a = static_cast<const double2*>(col_ptr(c0));
-                if (basis == BASIS_X)
-                    hipLaunchKernelGGL((k_norm_partial_basis<double2, BASIS_X>), grid, dim3(kBlock), 0, stream_, a,
-                                       dim(), nwork, p, part + c0 * kNormBlocks);
-                else
-                    hipLaunchKernelGGL((k_norm_partial_basis<double2, BASIS_Y>), grid, dim3(kBlock), 0, stream_, a,
-                                       dim(), nwork, p, part + c0 * kNormBlocks);
-            } else {
-                const float2* a = static_cast<const float2*>(col_ptr(c0));
-                if (basis == BASIS_X)
-                    hipLaunchKernelGGL((k_norm_partial_basis<float2, BASIS_X>), grid, dim3(kBlock), 0, stream_, a,
-                                       dim(), nwork, p, part + c0 * kNormBlocks);
-                else
-                    hipLaunchKernelGGL((k_norm_partial_basis<float2, BASIS_Y>), grid, dim3(kBlock), 0, stream_, a,
-                                       dim(), nwork, p, part + c0 * kNormBlocks);
-            }
-            HIP_OK(hipGetLastError());
-            continue;
-        }
-        if (prec_ == PREC_C128)
-            hipLaunchKernelGGL(k_norm_partial<double2>, dim3(kNormBlocks, ny), dim3(kBlock), 0, stream_,
-                               static_cast<const double2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * kNormBlocks);
-        else
-            hipLaunchKernelGGL(k_norm_partial<float2>, dim3(kNormBlocks, ny), dim3(kBlock), 0, stream_,
-                               static_cast<const float2*>(col_ptr(c0)), dim(), nwork, pl, part + c0 * kNormBlocks);
-        HIP_OK(hipGetLastError());
-    }
-}
-
-// w0 per column from the partials, summed in column-major block order.
-std::vector<double> GpuVectorState::w0_from_partials(size_t qbit, const double* h, int basis) const
-{
-    const size_t nc = counts_.size();
-    std::vector<double> w0(nc, 0.0);
-    const int p = int(nr_bits_ - 1 - qbit);
-    if (basis_) {
-        // measured in the basis the columns were collapsed in, a basis
-        // vector gives its own bit; in another basis either outcome with
-        // probability 1/2
-        for (size_t c = 0; c < nc; ++c)
-            w0[c] = basis != basis_rot_ ? 0.5 : ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
-        return w0;
-    }
-    for (size_t c = 0; c < nc; ++c)
-        for (unsigned b = 0; b < kNormBlocks; ++b) w0[c] += h[c * kNormBlocks + b];
-    return w0;
-}
-
-std::vector<double> GpuVectorState::marginal_w0(size_t qbit, int basis)
-{
-    check_qbit(qbit);
-    check_basis(basis);
-    flush();
-    if (basis_) return w0_from_partials(qbit, nullptr, basis);
-    const size_t nc = counts_.size();
-    double* part = partial_buf(size_t(kNormBlocks) * nc);
-    launch_marginal(qbit, part, basis);
-    std::vector<double> h(size_t(kNormBlocks) * nc);
-    HIP_OK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    HIP_OK(hipStreamSynchronize(stream_));
-    return w0_from_partials(qbit, h.data(), basis);
-}
-
 void GpuVectorState::check_basis(int basis)
 {
     if (basis != BASIS_Z && basis != BASIS_X && basis != BASIS_Y)
         throw Error(ERR_INVALID_ARGUMENT, "Unknown measurement basis " + std::to_string(basis));
 }
 
-size_t GpuVectorState::marginal_partials() const { return size_t(kNormBlocks) * counts_.size(); }
+// Measured in the basis the compact columns were collapsed in, a basis
+// vector gives its own bit; in another basis either outcome with
+// probability 1/2.
+std::vector<double> GpuVectorState::basis_w0(size_t qbit, int basis) const
+{
+    const int p = int(nr_bits_ - 1 - qbit);
+    std::vector<double> w0(counts_.size());
+    for (size_t c = 0; c < counts_.size(); ++c)
+        w0[c] = basis != basis_rot_ ? 0.5 : ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+    return w0;
+}
 
-void GpuVectorState::marginal_w0_async(size_t qbit, double* host, int basis)
+size_t GpuVectorState::marginal_enqueue(ItemQueue& q, size_t qbit, int basis)
 {
     check_qbit(qbit);
     check_basis(basis);
     flush();
-    if (basis_) return;
-    double* part = partial_buf(marginal_partials());
-    launch_marginal(qbit, part, basis);
-    HIP_OK(hipMemcpyAsync(host, part, marginal_partials() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (basis_) return SIZE_MAX;
+    const int p = int(nr_bits_ - 1 - qbit);
+    size_t first = SIZE_MAX;
+    for (size_t c = 0; c < counts_.size(); ++c) {
+        const size_t it = q.add_marginal(MargItem{col_ptr(c), dim() / 2, p, basis});
+        if (c == 0) first = it;
+    }
+    return first;
 }
 
-std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const double* host, int basis) const
+std::vector<double> GpuVectorState::marginal_collect(const ItemQueue& q, size_t first, size_t qbit, int basis) const
 {
-    return w0_from_partials(qbit, host, basis);
+    if (first == SIZE_MAX) return basis_w0(qbit, basis);
+    std::vector<double> w0(counts_.size());
+    for (size_t c = 0; c < counts_.size(); ++c) w0[c] = q.w0(first + c);
+    return w0;
+}
+
+std::vector<double> GpuVectorState::marginal_w0(size_t qbit, int basis)
+{
+    ItemQueue& q = iq();
+    const size_t first = marginal_enqueue(q, qbit, basis);
+    if (first == SIZE_MAX) return basis_w0(qbit, basis);
+    q.launch(stream_);
+    HIP_OK(hipStreamSynchronize(stream_));
+    return marginal_collect(q, first, qbit, basis);
 }
 
 // ------------------------------------------- allocation-free marginal w0
@@ -1287,7 +1110,7 @@ std::vector<double> GpuVectorState::marginal_w0_collect(size_t qbit, const doubl
 // the library runs on holds its own copy until the process ends.  Every call
 // overwrites the entries it reads, so nothing needs clearing between calls.
 static constexpr size_t kNoAllocMaxCols = 256;
-static constexpr size_t kNoAllocW0 = kNoAllocMaxCols * kNormBlocks;
+static constexpr size_t kNoAllocW0 = kNoAllocMaxCols * size_t(kNormBlocks);
 static __device__ __align__(256) double scratch[kNoAllocW0 + kNoAllocMaxCols];
 
 // w0 of each column: the partials summed one after another from 0.0, the
@@ -1334,8 +1157,8 @@ void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
     if (!queue_.empty())
         throw Error(ERR_INVALID_ARGUMENT, "Gates are pending on the state: flush it before the allocation-free marginal");
     if (basis_) {
-        const int p = int(nr_bits_ - 1 - qbit);
-        for (size_t c = 0; c < nc; ++c) w0s[c] = basis_rot_ != 0 ? 0.5 : ((basis_idx_[c] >> p) & 1) ? 0.0 : 1.0;
+        const std::vector<double> w = basis_w0(qbit, BASIS_Z);
+        std::copy(w.begin(), w.end(), w0s);
         return;
     }
     if (nc > kNoAllocMaxCols)
@@ -1349,7 +1172,8 @@ void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
                                               ", but device " + std::to_string(dev) + " is current");
     double* part = noalloc_scratch(dev);
     double* w = part + kNoAllocW0;
-    launch_marginal(qbit, part);   // one launch: nc <= 65535 columns
+    const int p = int(nr_bits_ - 1 - qbit);
+    launch_marginal_cols(prec_ == PREC_C64, col_ptr(0), dim(), uint32_t(nc), dim() / 2, p, BASIS_Z, part, stream_);
     hipLaunchKernelGGL(k_sum_partials, dim3(unsigned((nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream_, part,
                        kNormBlocks, unsigned(nc), w);
     HIP_OK(hipGetLastError());
@@ -1357,8 +1181,11 @@ void GpuVectorState::marginal_w0_noalloc(size_t qbit, double* w0s, size_t len)
     HIP_OK(hipStreamSynchronize(stream_));
 }
 
-void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
-                                     uint64_t* res, size_t res_len, bool collapse, int basis)
+// The bits written and the collapse of measure_into (vectorstate.rs:237-329)
+// for draws n0s already made: runs of column c with index < n0s[c] read 0.
+void GpuVectorState::collapse_enqueue(ItemQueue& q, size_t qbit, size_t cbit, const size_t* n0s,
+                                      const std::vector<double>& w0s, uint64_t* res, size_t res_len, bool collapse,
+                                      int basis)
 {
     check_qbit(qbit);
     check_basis(basis);
@@ -1376,52 +1203,18 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
     if (!collapse) return;
     const int p = int(nr_bits_ - 1 - qbit);
     if (basis_ && new_nr == nc && basis == basis_rot_) return;   // a basis column collapses onto itself
-    materialize();
+    materialize_enqueue(q);
     const uint64_t nwork = dim() / 2;
-    auto launch = [&](const void* src, void* d0, void* d1, double s0, double s1) {
-        if (basis != BASIS_Z) {
-            const dim3 g(grid_for(nwork));
-            if (prec_ == PREC_C128) {
-                const double2* a = static_cast<const double2*>(src);
-                double2 *o0 = static_cast<double2*>(d0), *o1 = static_cast<double2*>(d1);
-                if (basis == BASIS_X)
-                    hipLaunchKernelGGL((k_collapse_basis<double2, BASIS_X>), g, dim3(kBlock), 0, stream_, a, o0, o1,
-                                       nwork, p, s0, s1);
-                else
-                    hipLaunchKernelGGL((k_collapse_basis<double2, BASIS_Y>), g, dim3(kBlock), 0, stream_, a, o0, o1,
-                                       nwork, p, s0, s1);
-            } else {
-                const float2* a = static_cast<const float2*>(src);
-                float2 *o0 = static_cast<float2*>(d0), *o1 = static_cast<float2*>(d1);
-                if (basis == BASIS_X)
-                    hipLaunchKernelGGL((k_collapse_basis<float2, BASIS_X>), g, dim3(kBlock), 0, stream_, a, o0, o1,
-                                       nwork, p, float(s0), float(s1));
-                else
-                    hipLaunchKernelGGL((k_collapse_basis<float2, BASIS_Y>), g, dim3(kBlock), 0, stream_, a, o0, o1,
-                                       nwork, p, float(s0), float(s1));
-            }
-            HIP_OK(hipGetLastError());
-            return;
-        }
-        if (prec_ == PREC_C128)
-            hipLaunchKernelGGL(k_collapse<double2>, dim3(grid_for(nwork)), dim3(kBlock), 0, stream_,
-                               static_cast<const double2*>(src), static_cast<double2*>(d0), static_cast<double2*>(d1),
-                               nwork, p, s0, s1);
-        else
-            hipLaunchKernelGGL(k_collapse<float2>, dim3(grid_for(nwork)), dim3(kBlock), 0, stream_,
-                               static_cast<const float2*>(src), static_cast<float2*>(d0), static_cast<float2*>(d1),
-                               nwork, p, float(s0), float(s1));
-        HIP_OK(hipGetLastError());
+    auto add = [&](const void* src, void* d0, void* d1, double s0, double s1) {
+        q.add_collapse(CollItem{src, d0, d1, nwork, s0, s1, p, basis, 0});
     };
     std::vector<size_t> ncounts;
     if (new_nr == nc) {
         for (size_t c = 0; c < nc; ++c) {
             const size_t n0 = std::min(n0s[c], counts_[c]);
             void* col = col_ptr(c);
-            if (n0 == counts_[c])
-                launch(col, col, nullptr, 1.0 / std::sqrt(w0s[c]), 0.0);
-            else
-                launch(col, nullptr, col, 0.0, 1.0 / std::sqrt(1.0 - w0s[c]));
+            if (n0 == counts_[c]) add(col, col, nullptr, 1.0 / std::sqrt(w0s[c]), 0.0);
+            else add(col, nullptr, col, 0.0, 1.0 / std::sqrt(1.0 - w0s[c]));
             ncounts.push_back(counts_[c]);
         }
     } else {
@@ -1432,25 +1225,34 @@ void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s
             const size_t n0 = std::min(n0s[c], counts_[c]), count = counts_[c];
             const double w0 = w0s[c];
             if (n0 == count) {
-                launch(col_ptr(c), fcol(ni), nullptr, 1.0 / std::sqrt(w0), 0.0);
+                add(col_ptr(c), fcol(ni), nullptr, 1.0 / std::sqrt(w0), 0.0);
                 ncounts.push_back(count);
                 ni += 1;
             } else if (n0 == 0) {
-                launch(col_ptr(c), nullptr, fcol(ni), 0.0, 1.0 / std::sqrt(1.0 - w0));
+                add(col_ptr(c), nullptr, fcol(ni), 0.0, 1.0 / std::sqrt(1.0 - w0));
                 ncounts.push_back(count);
                 ni += 1;
             } else {
-                launch(col_ptr(c), fcol(ni), fcol(ni + 1), 1.0 / std::sqrt(w0), 1.0 / std::sqrt(1.0 - w0));
+                add(col_ptr(c), fcol(ni), fcol(ni + 1), 1.0 / std::sqrt(w0), 1.0 / std::sqrt(1.0 - w0));
                 ncounts.push_back(n0);
                 ncounts.push_back(count - n0);
                 ni += 2;
             }
         }
-        dev_release(amp_, amp_cols_ * dim() * elem_bytes());
+        // the old columns are read by the queued collapses
+        q.release_after(amp_, amp_cols_ * dim() * elem_bytes(), stream_);
         amp_ = fresh;
         amp_cols_ = new_nr;
     }
     counts_ = ncounts;
+}
+
+void GpuVectorState::measure_with_n0(size_t qbit, size_t cbit, const size_t* n0s, const std::vector<double>& w0s,
+                                     uint64_t* res, size_t res_len, bool collapse, int basis)
+{
+    ItemQueue& q = iq();
+    collapse_enqueue(q, qbit, cbit, n0s, w0s, res, res_len, collapse, basis);
+    q.launch(stream_);
 }
 
 void GpuVectorState::measure_into(size_t qbit, size_t cbit, uint64_t* res, size_t res_len, RngSource& rng,
@@ -1488,27 +1290,14 @@ static TreeDesc tree_shape(uint64_t dim, uint64_t* entries)
     return td;
 }
 
-size_t GpuVectorState::measure_all_host_bytes() const
-{
-    return nr_shots_ * (sizeof(double) + sizeof(uint64_t)) + counts_.size() * 32 * sizeof(double) +
-           (counts_.size() + 1) * sizeof(uint64_t);
-}
-
-// Host scratch of measure_all_async: [uniforms: nr_shots doubles][indices:
-// nr_shots u64][per column: 32 doubles holding the top tree level (or the
-// amplitudes of a state of at most 16 entries)][first run of each column,
-// nc + 1 u64].  Every launch covers all columns: one per tree level, one 2-D
-// copy of the top levels, one sampling launch.
-void GpuVectorState::measure_all_async(void* host)
+// The sampling of measure_all_into_helper (vectorstate.rs:106-161): the
+// tree levels of every column and one walk per run, queued.
+size_t GpuVectorState::sample_enqueue(ItemQueue& q, const double* u)
 {
     flush();
-    if (basis_ && basis_rot_ != 0) materialize();   // rotated basis columns are dense in Z
-    if (basis_) return;
+    if (basis_ && basis_rot_ != 0) materialize_enqueue(q);   // rotated basis columns are dense in Z
+    if (basis_) return SIZE_MAX;
     const size_t nc = counts_.size();
-    double* hu = static_cast<double*>(host);
-    uint64_t* hidx = reinterpret_cast<uint64_t*>(hu + nr_shots_);
-    double* htop = reinterpret_cast<double*>(hidx + nr_shots_);
-    uint64_t* hfirst = reinterpret_cast<uint64_t*>(htop + 32 * nc);
     uint64_t per_col = 0;
     TreeDesc td = tree_shape(dim(), &per_col);
     const size_t need = std::max<size_t>(per_col * nc, 1);
@@ -1526,104 +1315,49 @@ void GpuVectorState::measure_all_async(void* host)
             o += td.size[l];
         }
     }
+    std::vector<LevelItem> levels;
+    levels.reserve(size_t(td.nlev - 1) * nc);
+    for (int l = 1; l < td.nlev; ++l)
+        for (size_t c = 0; c < nc; ++c)
+            levels.push_back(LevelItem{l == 1 ? static_cast<const void*>(col_ptr(c))
+                                              : static_cast<const void*>(td.lev[l - 1] + c * per_col),
+                                       const_cast<double*>(td.lev[l]) + c * per_col, td.size[l - 1], 0});
+    std::vector<uint64_t> first(nc + 1);
     uint64_t run = 0;
     for (size_t c = 0; c < nc; ++c) {
-        hfirst[c] = run;
+        first[c] = run;
         run += counts_[c];
     }
-    hfirst[nc] = run;
-    const size_t ub = nr_shots_ * sizeof(double), fb = (nc + 1) * sizeof(uint64_t);
-    char* buf = static_cast<char*>(small_buf(2 * ub + fb + 64));
-    double* du = reinterpret_cast<double*>(buf);
-    uint64_t* dout = reinterpret_cast<uint64_t*>(buf + ub);
-    uint64_t* dfirst = reinterpret_cast<uint64_t*>(buf + 2 * ub);
-    HIP_OK(hipMemcpyAsync(du, hu, ub, hipMemcpyHostToDevice, stream_));
-    HIP_OK(hipMemcpyAsync(dfirst, hfirst, fb, hipMemcpyHostToDevice, stream_));
-    const uint64_t cstride = dim();
-    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
-        const unsigned ny = unsigned(std::min<size_t>(65535, nc - c0));
-        for (int l = 1; l < td.nlev; ++l) {
-            const uint64_t chunks =
-                (td.size[l - 1] + uint64_t(kBlock) * kTreeFanout - 1) / (uint64_t(kBlock) * kTreeFanout);
-            const dim3 grid(unsigned(std::min<uint64_t>(chunks, 65536)), ny);
-            double* outp = const_cast<double*>(td.lev[l]) + per_col * c0;
-            if (l == 1) {
-                if (prec_ == PREC_C128)
-                    hipLaunchKernelGGL(k_tree_level<double2>, grid, dim3(kBlock), 0, stream_,
-                                       static_cast<const double2*>(col_ptr(c0)), cstride, td.size[0], outp, per_col);
-                else
-                    hipLaunchKernelGGL(k_tree_level<float2>, grid, dim3(kBlock), 0, stream_,
-                                       static_cast<const float2*>(col_ptr(c0)), cstride, td.size[0], outp, per_col);
-            } else {
-                hipLaunchKernelGGL(k_tree_level<double>, grid, dim3(kBlock), 0, stream_, td.lev[l - 1] + per_col * c0,
-                                   per_col, td.size[l - 1], outp, per_col);
-            }
-            HIP_OK(hipGetLastError());
-        }
-    }
-    // the top levels (or the amplitudes themselves) for the norm check
-    const int top = td.nlev - 1;
-    if (nc) {
-        if (top == 0)
-            HIP_OK(hipMemcpy2DAsync(htop, 32 * sizeof(double), col_ptr(0), size_t(cstride) * elem_bytes(),
-                                    size_t(td.size[0]) * elem_bytes(), nc, hipMemcpyDeviceToHost, stream_));
-        else
-            HIP_OK(hipMemcpy2DAsync(htop, 32 * sizeof(double), td.lev[top], size_t(per_col) * sizeof(double),
-                                    size_t(td.size[top]) * sizeof(double), nc, hipMemcpyDeviceToHost, stream_));
-    }
-    // sampling is in bounds for any weights (an all-zero group walks to its
-    // first child); invalid norms are reported by measure_all_finish
-    if (run) {
-        const unsigned g = unsigned((run + kBlock - 1) / kBlock);
-        if (prec_ == PREC_C128)
-            hipLaunchKernelGGL(k_tree_sample<double2>, dim3(g), dim3(kBlock), 0, stream_,
-                               static_cast<const double2*>(col_ptr(0)), cstride, td, per_col, dfirst, int(nc), du,
-                               run, dout);
-        else
-            hipLaunchKernelGGL(k_tree_sample<float2>, dim3(g), dim3(kBlock), 0, stream_,
-                               static_cast<const float2*>(col_ptr(0)), cstride, td, per_col, dfirst, int(nc), du,
-                               run, dout);
-        HIP_OK(hipGetLastError());
-    }
-    HIP_OK(hipMemcpyAsync(hidx, dout, nr_shots_ * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+    first[nc] = run;
+    SampleJob job{};
+    job.a0 = col_ptr(0);
+    job.col_stride = dim();
+    job.td = td;
+    job.per_col = per_col;
+    job.ncols = int32_t(nc);
+    return q.add_sample(job, levels, first.data(), u, size_t(run));
 }
 
 // Grouping and bit mapping of measure_all_into_helper (vectorstate.rs:106-161)
-// on the indices measure_all_async sampled; outcomes grouped per column in
-// ascending index order.
-void GpuVectorState::measure_all_finish(const size_t* cbits, size_t ncbits, const void* host, uint64_t* res,
-                                        size_t res_len, bool collapse, int rot)
+// on the sampled indices; outcomes grouped per column in ascending index
+// order.
+void GpuVectorState::sample_finish(const ItemQueue& q, size_t job, const size_t* cbits, size_t ncbits, uint64_t* res,
+                                   size_t res_len, bool collapse, int rot)
 {
     check_basis(rot);
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
     const size_t nc = counts_.size();
-    const uint64_t* hidx = reinterpret_cast<const uint64_t*>(static_cast<const double*>(host) + nr_shots_);
-    const double* htop = reinterpret_cast<const double*>(hidx + nr_shots_);
     std::vector<uint64_t> uidx, ucnt, samples;
+    const uint64_t* hidx = nullptr;
     if (!basis_) {
+        if (job == SIZE_MAX) throw Error(ERR_INVALID_ARGUMENT, "measure_all: no sampling was queued for dense columns");
+        hidx = q.indices(job);
         // all-zero or NaN weights make WeightedIndex::new fail (vectorstate.rs:123-125)
-        uint64_t per_col = 0;
-        const TreeDesc td = tree_shape(dim(), &per_col);
-        const int top = td.nlev - 1;
-        for (size_t c = 0; c < nc; ++c) {
-            if (counts_[c] == 0) continue;
-            double total = 0.0;
-            const size_t nt = size_t(td.size[top]);
-            for (size_t k = 0; k < nt; ++k) {
-                if (top > 0) {
-                    total += htop[32 * c + k];
-                } else if (prec_ == PREC_C128) {
-                    const double re = htop[32 * c + 2 * k], im = htop[32 * c + 2 * k + 1];
-                    total += re * re + im * im;
-                } else {
-                    const float* f = reinterpret_cast<const float*>(htop + 32 * c);
-                    total += double(f[2 * k]) * f[2 * k] + double(f[2 * k + 1]) * f[2 * k + 1];
-                }
-            }
-            if (!(total > 0.0) || !std::isfinite(total))
+        const double* tops = q.tops(job);
+        for (size_t c = 0; c < nc; ++c)
+            if (counts_[c] != 0 && !(tops[c] > 0.0 && std::isfinite(tops[c])))
                 throw Error(ERR_INVALID_WEIGHTS, "Cannot sample from a state with zero or invalid norm");
-        }
     }
     size_t run = 0;
     for (size_t c = 0; c < nc; ++c) {
@@ -1666,13 +1400,16 @@ void GpuVectorState::measure_all_finish(const size_t* cbits, size_t ncbits, cons
 void GpuVectorState::measure_all_uniforms(const size_t* cbits, size_t ncbits, const double* u01, uint64_t* res,
                                           size_t res_len, bool collapse, int rot)
 {
+    check_basis(rot);
     if (res_len < nr_shots_) throw err_not_enough_space(res_len, nr_shots_);
     if (ncbits != nr_bits_) throw err_invalid_nr_measurement_bits(ncbits, nr_bits_);
-    std::vector<double> host((measure_all_host_bytes() + sizeof(double) - 1) / sizeof(double));
-    std::copy(u01, u01 + nr_shots_, host.data());
-    measure_all_async(host.data());
-    HIP_OK(hipStreamSynchronize(stream_));
-    measure_all_finish(cbits, ncbits, host.data(), res, res_len, collapse, rot);
+    ItemQueue& q = iq();
+    const size_t job = sample_enqueue(q, u01);
+    if (!q.empty()) {
+        q.launch(stream_);
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+    sample_finish(q, job, cbits, ncbits, res, res_len, collapse, rot);
 }
 
 void GpuVectorState::measure_all_into(const size_t* cbits, size_t ncbits, uint64_t* res, size_t res_len,

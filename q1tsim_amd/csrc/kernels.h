@@ -18,6 +18,7 @@ namespace q1t {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 waves)
 constexpr int kTreeFanout = 16;   // F of the canonical sampling tree
+constexpr unsigned kNormBlocks = 256;   // partial sums per column of a one-qubit marginal
 
 // Sorted (ascending) index-bit positions at which zero bits are deposited.
 struct PosList {

@@ -78,13 +78,26 @@ struct DeviceBuffer {
     ~DeviceBuffer();
 };
 
+// Pinned host buffer that grows on demand; the caller makes sure no copy
+// from it is in flight when it grows or is rewritten.
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes);
+    ~PinnedBuffer();
+};
+
 // Flush the pending gates of many states together (all of one precision,
 // tile geometry and fusion setting, sharing `stream`): states at least
 // M+2 qubits wide are planned into sweeps and the r-th sweep of every state
 // runs in one launch over all their tiles; smaller states run whole in LDS,
 // one workgroup per column, all in one launch.
+// The table goes up from `stage` (the call ends with the stream
+// synchronised, so the next call may rewrite it).  `plan_s` and `pack_s`
+// (optional) accumulate the host planning and table-building time.
 void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
-                       RunStats& agg, unsigned tile_chunk);
+                       PinnedBuffer& stage, ItemQueue& iq, RunStats& agg, unsigned tile_chunk,
+                       double* plan_s = nullptr, double* pack_s = nullptr);
 
 // Sources of the kernels specialised to the sweeps of the plan a flush of
 // `q` would run (plain programs; with fusion_k > 0 greedy fixed-width

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -48,10 +49,14 @@ k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 }
 
 // Sweeps of many states in one launch: item i covers flattened tiles
-// [first_tile[i], first_tile[i+1]) of its own column with its own program.
+// [first_tile[i], first_tile[i+1]) of one state, its columns one after the
+// other (column c at amp + c * col_stride, 2^log_ntiles tiles each), with the
+// state's program.
 struct SweepItem {
     void* amp;
     uint64_t first_tile;
+    uint64_t col_stride;
+    uint32_t ncols, log_ntiles;
     SweepArgs sa;
 };
 
@@ -77,7 +82,9 @@ k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_
         for (uint64_t t = t0; t < t1; ++t) {
             while (lo + 1 < nitems && items[lo + 1].first_tile <= t) ++lo;
             const SweepItem& it = items[lo];
-            sweep_tile<V, M, R, SW, WIDE>(static_cast<V*>(it.amp), it.sa, t - it.first_tile, lds);
+            const uint64_t lt = t - it.first_tile;
+            V* a = static_cast<V*>(it.amp) + (lt >> it.log_ntiles) * it.col_stride;
+            sweep_tile<V, M, R, SW, WIDE>(a, it.sa, lt & ((uint64_t(1) << it.log_ntiles) - 1), lds);
         }
     }
 }
@@ -573,6 +580,24 @@ DeviceBuffer::~DeviceBuffer()
     if (ptr) (void)hipFree(ptr);
 }
 
+void* PinnedBuffer::get(size_t bytes)
+{
+    if (bytes > cap) {
+        if (ptr) SW_HIP_OK(hipHostFree(ptr));
+        ptr = nullptr;
+        cap = 0;
+        const size_t c = std::max<size_t>(bytes, size_t(1) << 20) * 3 / 2;
+        SW_HIP_OK(hipHostMalloc(&ptr, c, hipHostMallocDefault));
+        cap = c;
+    }
+    return ptr;
+}
+
+PinnedBuffer::~PinnedBuffer()
+{
+    if (ptr) (void)hipHostFree(ptr);
+}
+
 // A gate of a small state, applied in LDS.
 struct SmallOp {
     int32_t kind;      // 0: dense on nt targets, 1: diagonal on nt targets, 2: phase on cmask, 3: swap t0/t1
@@ -742,15 +767,18 @@ static SmallOp make_small_op(const GateOp& g, std::vector<double2>& vals)
 }
 
 void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t stream, DeviceBuffer& prog,
-                       RunStats& agg, unsigned tile_chunk)
+                       PinnedBuffer& stage, ItemQueue& iq, RunStats& agg, unsigned tile_chunk, double* plan_s,
+                       double* pack_s)
 {
+    const auto t_plan = std::chrono::steady_clock::now();
     if (tile_chunk == 0) throw Error(ERR_INVALID_ARGUMENT, "tile chunk must be at least 1");
     std::vector<GpuVectorState*> live;
     for (GpuVectorState* s : states)
         if (!s->pending().empty()) {
-            s->prepare_batched_flush();
+            s->prepare_batched_flush(iq);
             live.push_back(s);
         }
+    if (!iq.empty()) iq.launch(stream);   // compact columns written out before the sweeps
     if (live.empty()) return;
     const Precision prec = live[0]->precision();
     const TileGeom g = live[0]->geometry();
@@ -833,29 +861,30 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         for (const Plan& p : plans)
             if (!p.error.empty()) throw Error(ERR_INVALID_ARGUMENT, "Batch planning failed: " + p.error);
     }
-    std::vector<char> host;
-    std::vector<float> blocks;
-    std::vector<SmallOp> sops;
-    std::vector<double2> svals;
-    auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
-    size_t max_steps = 0;
+    if (plan_s) *plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count();
+    const auto t_pack = std::chrono::steady_clock::now();
+    // 2. one table in pinned memory, uploaded in one copy: every state's
+    // phases and ops, the block matrices, the small-state ops and items, and
+    // per round one sweep item per state (all its columns).  Offsets first
+    // (serial arithmetic), then the states fill their parts in parallel.
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t off = 0, max_steps = 0, nblock_floats = 0, nsops = 0, nsvals = 0;
     int max_small_n = 0;
+    std::vector<size_t> bbase(live.size(), 0), sbase(live.size(), 0), vbase(live.size(), 0);
     for (size_t i = 0; i < live.size(); ++i) {
         Plan& p = plans[i];
         if (p.small) {
-            p.sop_begin = int(sops.size());
-            const uint32_t vbase = uint32_t(svals.size());
-            for (SmallOp o : p.sops) {
-                o.moff += vbase;
-                sops.push_back(o);
-            }
-            svals.insert(svals.end(), p.svals.begin(), p.svals.end());
-            p.sop_end = int(sops.size());
+            sbase[i] = nsops;
+            vbase[i] = nsvals;
+            p.sop_begin = int(nsops);
+            nsops += p.sops.size();
+            nsvals += p.svals.size();
+            p.sop_end = int(nsops);
             max_small_n = std::max(max_small_n, int(live[i]->nr_bits()));
             continue;
         }
-        const uint32_t bbase = uint32_t(blocks.size());
-        blocks.insert(blocks.end(), p.blocks.begin(), p.blocks.end());
+        bbase[i] = nblock_floats;
+        nblock_floats += p.blocks.size();
         p.poff.assign(p.steps.size(), 0);
         p.ooff.assign(p.steps.size(), 0);
         p.nph.assign(p.steps.size(), 0);
@@ -863,107 +892,138 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
             p.wide[s] = has_wide_blocks(p.ph[s]);
-            for (SwPhase& ph : p.ph[s])
-                if (ph.kind == 1) ph.op_begin += bbase;
-            align();
-            p.poff[s] = host.size();
-            host.insert(host.end(), reinterpret_cast<const char*>(p.ph[s].data()),
-                        reinterpret_cast<const char*>(p.ph[s].data() + p.ph[s].size()));
-            align();
-            p.ooff[s] = host.size();
-            host.insert(host.end(), reinterpret_cast<const char*>(p.ops[s].data()),
-                        reinterpret_cast<const char*>(p.ops[s].data() + p.ops[s].size()));
+            p.poff[s] = off;
+            off = up(off + p.ph[s].size() * sizeof(SwPhase));
+            p.ooff[s] = off;
+            off = up(off + p.ops[s].size() * sizeof(SwOp));
             p.nph[s] = int(p.ph[s].size());
         }
         max_steps = std::max(max_steps, p.steps.size());
     }
-    align();
-    const size_t boff = host.size();
-    host.insert(host.end(), reinterpret_cast<const char*>(blocks.data()),
-                reinterpret_cast<const char*>(blocks.data() + blocks.size()));
-    align();
-    const size_t soff = host.size();
-    host.insert(host.end(), reinterpret_cast<const char*>(sops.data()),
-                reinterpret_cast<const char*>(sops.data() + sops.size()));
-    align();
-    const size_t voff = host.size();
-    host.insert(host.end(), reinterpret_cast<const char*>(svals.data()),
-                reinterpret_cast<const char*>(svals.data() + svals.size()));
-    align();
-    // small-state items
-    std::vector<SmallItem> sitems;
-    for (size_t i = 0; i < live.size(); ++i) {
-        if (!plans[i].small || plans[i].sop_end == plans[i].sop_begin) continue;
-        for (size_t c = 0; c < live[i]->nr_cols(); ++c)
-            sitems.push_back(SmallItem{live[i]->col_ptr(c), int(live[i]->nr_bits()), plans[i].sop_begin,
-                                       plans[i].sop_end, 0});
-    }
-    const size_t sioff = host.size();
-    host.insert(host.end(), reinterpret_cast<const char*>(sitems.data()),
-                reinterpret_cast<const char*>(sitems.data() + sitems.size()));
-    align();
-    // sweep items of every round, built against the device base address
-    const size_t items_off = host.size();
-    size_t n_items_total = 0;
+    const size_t boff = off;
+    off = up(off + nblock_floats * sizeof(float));
+    const size_t soff = off;
+    off = up(off + nsops * sizeof(SmallOp));
+    const size_t voff = off;
+    off = up(off + nsvals * sizeof(double2));
+    size_t nsitems = 0;
     for (size_t i = 0; i < live.size(); ++i)
-        for (size_t s = 0; s < plans[i].steps.size(); ++s)
-            if (plans[i].steps[s].sweep) n_items_total += live[i]->nr_cols();
-    const size_t total = items_off + n_items_total * sizeof(SweepItem);
-    char* dbase = static_cast<char*>(prog.get(total, stream));
-    host.resize(total);
+        if (plans[i].small && plans[i].sop_end != plans[i].sop_begin) nsitems += live[i]->nr_cols();
+    const size_t sioff = off;
+    off = up(off + nsitems * sizeof(SmallItem));
+    // sweep items: round-major; item (r, i) at item_at[r][i], tiles from
+    // round_first[r][i]
     std::vector<std::pair<size_t, size_t>> round_items(max_steps);   // (first item, count)
-    size_t wi = 0;
     std::vector<uint64_t> round_tiles(max_steps, 0);
     std::vector<char> round_wide(max_steps, 0);
+    std::vector<std::vector<uint64_t>> item_idx(max_steps), item_tile(max_steps);
+    size_t wi = 0;
     for (size_t r = 0; r < max_steps; ++r) {
         round_items[r].first = wi;
+        item_idx[r].assign(live.size(), SIZE_MAX);
+        item_tile[r].assign(live.size(), 0);
         uint64_t tiles = 0;
         for (size_t i = 0; i < live.size(); ++i) {
             const Plan& p = plans[i];
-            if (r >= p.steps.size() || !p.steps[r].sweep) continue;
+            if (p.small || r >= p.steps.size() || !p.steps[r].sweep) continue;
             round_wide[r] |= p.wide[r];
-            const int n = int(live[i]->nr_bits());
-            for (size_t c = 0; c < live[i]->nr_cols(); ++c) {
-                SweepItem it;
-                std::memset(&it, 0, sizeof(it));
-                it.amp = live[i]->col_ptr(c);
-                it.first_tile = tiles;
-                it.sa.ntiles = uint64_t(1) << (n - g.M);
-                it.sa.tset.cnt = g.M;
-                for (int j = 0; j < g.M; ++j) {
-                    it.sa.tset.s[j] = p.steps[r].tile[j];
-                    it.sa.tpos[j] = p.steps[r].tile[j];
-                }
-                it.sa.phases = reinterpret_cast<const SwPhase*>(dbase + p.poff[r]);
-                it.sa.ops = reinterpret_cast<const SwOp*>(dbase + p.ooff[r]);
-                it.sa.blocks = reinterpret_cast<const float*>(dbase + boff);
-                it.sa.nphases = p.nph[r];
-                std::memcpy(host.data() + items_off + wi * sizeof(SweepItem), &it, sizeof(it));
-                tiles += it.sa.ntiles;
-                ++wi;
-            }
+            item_idx[r][i] = wi++;
+            item_tile[r][i] = tiles;
+            tiles += uint64_t(live[i]->nr_cols()) << (live[i]->nr_bits() - size_t(g.M));
         }
         round_items[r].second = wi - round_items[r].first;
         round_tiles[r] = tiles;
     }
-    SW_HIP_OK(hipMemcpyAsync(dbase, host.data(), total, hipMemcpyHostToDevice, stream));
+    const size_t items_off = off;
+    const size_t total = up(items_off + wi * sizeof(SweepItem));
+    char* dbase = static_cast<char*>(prog.get(total, stream));
+    char* host = static_cast<char*>(stage.get(total));
+    {
+        // small-state items in state order
+        std::vector<size_t> si_first(live.size(), 0);
+        size_t k = 0;
+        for (size_t i = 0; i < live.size(); ++i) {
+            si_first[i] = k;
+            if (plans[i].small && plans[i].sop_end != plans[i].sop_begin) k += live[i]->nr_cols();
+        }
+        std::atomic<size_t> next{0};
+        auto fill = [&] {
+            for (size_t i = next++; i < live.size(); i = next++) {
+                Plan& p = plans[i];
+                GpuVectorState& st = *live[i];
+                if (p.small) {
+                    SmallOp* so = reinterpret_cast<SmallOp*>(host + soff) + sbase[i];
+                    for (size_t j = 0; j < p.sops.size(); ++j) {
+                        so[j] = p.sops[j];
+                        so[j].moff += uint32_t(vbase[i]);
+                    }
+                    std::memcpy(reinterpret_cast<double2*>(host + voff) + vbase[i], p.svals.data(),
+                                p.svals.size() * sizeof(double2));
+                    if (p.sop_end != p.sop_begin) {
+                        SmallItem* it = reinterpret_cast<SmallItem*>(host + sioff) + si_first[i];
+                        for (size_t c = 0; c < st.nr_cols(); ++c)
+                            it[c] = SmallItem{st.col_ptr(c), int(st.nr_bits()), p.sop_begin, p.sop_end, 0};
+                    }
+                    continue;
+                }
+                std::memcpy(reinterpret_cast<float*>(host + boff) + bbase[i], p.blocks.data(),
+                            p.blocks.size() * sizeof(float));
+                const int n = int(st.nr_bits());
+                for (size_t s = 0; s < p.steps.size(); ++s) {
+                    if (!p.steps[s].sweep) continue;
+                    SwPhase* ph = reinterpret_cast<SwPhase*>(host + p.poff[s]);
+                    for (size_t j = 0; j < p.ph[s].size(); ++j) {
+                        ph[j] = p.ph[s][j];
+                        if (ph[j].kind == 1) ph[j].op_begin += uint32_t(bbase[i]);
+                    }
+                    std::memcpy(host + p.ooff[s], p.ops[s].data(), p.ops[s].size() * sizeof(SwOp));
+                    SweepItem it;
+                    std::memset(&it, 0, sizeof(it));
+                    it.amp = st.col_ptr(0);
+                    it.col_stride = uint64_t(1) << n;
+                    it.ncols = uint32_t(st.nr_cols());
+                    it.log_ntiles = uint32_t(n - g.M);
+                    it.first_tile = item_tile[s][i];
+                    it.sa.ntiles = uint64_t(1) << (n - g.M);
+                    it.sa.tset.cnt = g.M;
+                    for (int j = 0; j < g.M; ++j) {
+                        it.sa.tset.s[j] = p.steps[s].tile[j];
+                        it.sa.tpos[j] = p.steps[s].tile[j];
+                    }
+                    it.sa.phases = reinterpret_cast<const SwPhase*>(dbase + p.poff[s]);
+                    it.sa.ops = reinterpret_cast<const SwOp*>(dbase + p.ooff[s]);
+                    it.sa.blocks = reinterpret_cast<const float*>(dbase + boff);
+                    it.sa.nphases = p.nph[s];
+                    std::memcpy(host + items_off + item_idx[s][i] * sizeof(SweepItem), &it, sizeof(it));
+                }
+            }
+        };
+        const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const size_t nth = live.size() < 64 ? 1 : std::min(hw, live.size() / 32);
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nth; ++t) pool.emplace_back(fill);
+        fill();
+        for (std::thread& t : pool) t.join();
+    }
+    SW_HIP_OK(hipMemcpyAsync(dbase, host, total, hipMemcpyHostToDevice, stream));
+    if (pack_s) *pack_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_pack).count();
 
     // 2. small states: one launch
-    if (!sitems.empty()) {
+    if (nsitems) {
         const size_t lds = size_t(live[0]->elem_bytes()) << max_small_n;
         if (prec == PREC_C128) {
             SW_HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_batch<double2>),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            hipLaunchKernelGGL(k_small_batch<double2>, dim3(unsigned(std::min<size_t>(sitems.size(), 65535))),
+            hipLaunchKernelGGL(k_small_batch<double2>, dim3(unsigned(std::min<size_t>(nsitems, 65535))),
                                dim3(256), lds, stream, reinterpret_cast<const SmallItem*>(dbase + sioff),
-                               int(sitems.size()), reinterpret_cast<const SmallOp*>(dbase + soff),
+                               int(nsitems), reinterpret_cast<const SmallOp*>(dbase + soff),
                                reinterpret_cast<const double2*>(dbase + voff));
         } else {
             SW_HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_batch<float2>),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            hipLaunchKernelGGL(k_small_batch<float2>, dim3(unsigned(std::min<size_t>(sitems.size(), 65535))),
+            hipLaunchKernelGGL(k_small_batch<float2>, dim3(unsigned(std::min<size_t>(nsitems, 65535))),
                                dim3(256), lds, stream, reinterpret_cast<const SmallItem*>(dbase + sioff),
-                               int(sitems.size()), reinterpret_cast<const SmallOp*>(dbase + soff),
+                               int(nsitems), reinterpret_cast<const SmallOp*>(dbase + soff),
                                reinterpret_cast<const double2*>(dbase + voff));
         }
         SW_HIP_OK(hipGetLastError());

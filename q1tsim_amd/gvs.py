@@ -59,7 +59,7 @@ def jit_stats():
 
 
 def set_device_cache_limit(nbytes=0):
-    """Cap on released device blocks kept for reuse (0: half of device
+    """Cap on released device blocks kept for reuse (0: 7/8 of device
     memory); lowering it frees the oldest cached blocks now."""
     check(lib().q1t_device_cache_limit(int(nbytes)))
 

@@ -416,7 +416,8 @@ class Batch(object):
         g = s.gates
         return {"seconds": s.seconds, "waves": s.waves, "rounds": s.rounds, "sync_ops": s.sync_ops,
                 "start_s": s.start_s, "queue_s": s.queue_s, "flush_s": s.flush_s, "measure_s": s.measure_s,
-                "other_s": s.other_s,
+                "other_s": s.other_s, "column_launches": s.column_launches, "solo_ops": s.solo_ops,
+                "plan_s": s.plan_s, "pack_s": s.pack_s,
                 "bytes": g.bytes, "launches": g.launches, "sweeps": g.sweeps, "gates": g.gates,
                 "blocks": g.blocks, "phases": g.phases}
 

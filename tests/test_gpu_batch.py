@@ -237,3 +237,19 @@ def test_block_cache_after_stream_teardown():
         assert device_cache_bytes() <= 1 << 30
     finally:
         set_device_cache_limit(0)
+
+
+def test_batch_column_work_is_batched():
+    """Every measurement, measure_all and conditional gate of the branching
+    workload runs as column work of the whole batch (one launch per kind per
+    pass): no op circuit by circuit, and a number of launches per round that
+    does not grow with the number of circuits; results as alone."""
+    wl = workloads.batch_workload(96, seed=5, qubits=(8, 16), depths=(4, 10), meas=(1, 3))
+    batch, circs = run_batch(wl, "c64")
+    st = batch.stats()
+    assert st["solo_ops"] == 0
+    assert 0 < st["column_launches"] <= 40 * st["rounds"]
+    for (n, nc, prog, s), c in list(zip(wl, circs))[:24]:
+        solo = make(n, nc, prog, s, "c64")
+        solo.execute(SHOTS)
+        assert np.array_equal(np.array(c.cstate(), dtype=np.uint64), np.array(solo.cstate(), dtype=np.uint64)), (n, s)
