@@ -432,13 +432,20 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             if (!jf)
                 for (const SwPhase& ph : progs[s].phases)
                     if (ph.kind == 2) throw Error(ERR_HIP, "A sweep program with tile factors needs its specialised kernel");
+            const int shape = g.M * 16 + g.R;
             if (st.precision() == PREC_C128) {
-                if (g.M == 12 && g.R == 3) slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf);
-                else slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf);
+                switch (shape) {
+                case 12 * 16 + 3: slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf); break;
+                case 12 * 16 + 4: slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf); break;
+                default: throw Error(ERR_INVALID_ARGUMENT, "No sweep kernel for this complex128 tile shape");
+                }
             } else {
-                if (g.M == 13) slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf);
-                else if (g.R == 3) slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf);
-                else slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf);
+                switch (shape) {
+                case 13 * 16 + 4: slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
+                case 12 * 16 + 3: slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
+                case 12 * 16 + 4: slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
+                default: throw Error(ERR_INVALID_ARGUMENT, "No sweep kernel for this complex64 tile shape");
+                }
             }
             if (st.recording() && col_end - col_begin <= 65535) {
                 float f[kSweepFeatures];
