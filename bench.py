@@ -399,6 +399,26 @@ def main():
         apply_circuit(plain, q)
         extra["qft30_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
         extra["qft30_sweeps"] = plain.stats()["sweeps"]
+        # the same QFT in complex64: per-gate sweeps, greedy 4-qubit fusion and
+        # the learned plan, against the complex128 result
+        qft64 = {}
+        for name, fk, mode in (("per_gate", 0, 0), ("greedy_k4", 4, 0), ("learned", 4, 1)):
+            s64 = GpuVectorState(n, 1, "c64")
+            s64.set_plan(fk, True)
+            s64.set_cost_model(cm, mode, 5)
+            s64.set_jit(True)
+            s64.set_random_normal(7)
+            apply_circuit(s64, q)   # warm-up: compiles the kernels
+            s64.set_random_normal(7)
+            t0 = time.perf_counter()
+            apply_circuit(s64, q)
+            dt = time.perf_counter() - t0
+            st = s64.stats()
+            infid, maxabs = compare(plain, s64)
+            qft64[name] = {"ms": round(1e3 * dt, 2), "sweeps": st["sweeps"], "blocks": st["blocks"],
+                           "infidelity_vs_c128": infid, "max_abs_vs_c128": maxabs}
+            del s64
+        extra["qft30_c64"] = qft64
         res = np.zeros(1 << 16, np.uint64)
         from q1tsim_amd.gvs import Xoshiro
         t0 = time.perf_counter()
