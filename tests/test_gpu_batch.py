@@ -253,3 +253,32 @@ def test_batch_column_work_is_batched():
         solo = make(n, nc, prog, s, "c64")
         solo.execute(SHOTS)
         assert np.array_equal(np.array(c.cstate(), dtype=np.uint64), np.array(solo.cstate(), dtype=np.uint64)), (n, s)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_batch_conditional_gates_on_split_columns(prec):
+    """A conditional gate controlled by a peeked bit: the runs of one column
+    disagree, so the column set changes and the gate runs circuit by circuit
+    (solo_ops); a conditional gate after a collapsing measurement runs as
+    column work.  Mixed in one batch with X/Y measurements, every classical
+    state equals the solo run's and the oracle's."""
+    n = 9
+    wl = []
+    for k in range(6):
+        ops = [("gate",) + tuple(o) for o in workloads.random_layered(n, 4, 300 + k)]
+        prog = ops[:20] + [("peek", k % n, n, "Z"), ("cgate", [n], 1, "rx", [(k + 2) % n], [0.4 + k]),
+                           ("measure", (k + 3) % n, n + 1, "X" if k % 2 else "Z"),
+                           ("cgate", [n + 1], 1, "x", [(k + 5) % n], [])]
+        prog += ops[20:] + [("measure_all", list(range(n)), "Z")]
+        wl.append((n, n + 2, prog, 900 + k))
+    batch, circs = run_batch(wl, prec)
+    st = batch.stats()
+    assert st["solo_ops"] >= 1 and st["column_launches"] > 0
+    for (n_, nc, prog, s), c in zip(wl, circs):
+        solo = make(n_, nc, prog, s, prec)
+        solo.execute(SHOTS)
+        got = np.array(c.cstate(), dtype=np.uint64)
+        assert np.array_equal(got, np.array(solo.cstate(), dtype=np.uint64)), s
+        r = make_ref(n_, nc, prog)
+        r.execute(SHOTS, s)
+        assert np.array_equal(got, r.cstate()), s

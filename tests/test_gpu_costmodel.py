@@ -5,7 +5,6 @@ timings, then choosing programs that leave the amplitudes as the plain
 complex128 run has them (to complex64 precision) and whose wall time is
 reported against fixed-width greedy fusion."""
 import json
-import os
 import time
 
 import numpy as np
@@ -160,16 +159,13 @@ def test_learned_plan_vs_greedy_walltime():
     recorded on 28-qubit states of other circuits): the learned plan's wall
     time (host planning included, median of 3 flushes, specialised kernels)
     is at least 5 % below greedy fixed-width fusion's and within 10 % of the
-    per-gate plan's; written to gpurun_out."""
+    per-gate plan's."""
     n = 28
     cm = costmodel.load()
     ops = workloads.random_pairs(n, 40, 12345)
     res = {name: flush_seconds(ops, n, cm, fk, mode)
            for name, fk, mode in (("greedy", 4, 0), ("per_gate", 0, 0), ("learned", 4, 1))}
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open("gpurun_out/learned_vs_greedy_n%d.json" % n, "w") as f:
-        json.dump({"n": n, "held_out": cm.meta["held_out"],
-                   "plans": {k: {"seconds": v[0], "sweeps": v[1], "blocks": v[2]} for k, v in res.items()}}, f)
+    print(json.dumps({"n": n, "plans": {k: {"seconds": v[0], "sweeps": v[1], "blocks": v[2]} for k, v in res.items()}}))
     greedy, per_gate, learned = res["greedy"][0], res["per_gate"][0], res["learned"][0]
     print("learned / greedy / per-gate wall time: %.4f / %.4f / %.4f s" % (learned, greedy, per_gate))
     assert learned <= 0.95 * greedy
