@@ -87,7 +87,8 @@ size_t q1t_gvs_nr_bits(const q1t_gvs* s);
 size_t q1t_gvs_nr_cols(const q1t_gvs* s);
 int q1t_gvs_counts(const q1t_gvs* s, size_t* out, size_t len);
 
-/* QuState::apply_gate (qustate.rs:8, vectorstate.rs:166-178). */
+/* QuState::apply_gate (qustate.rs:8, vectorstate.rs:166-178).  Gates on up to
+ * 12 qubits (wider ones return ERR_NOT_SUPPORTED). */
 int q1t_gvs_apply_gate(q1t_gvs* s, const double* mat, size_t nr_affected, const size_t* bits, size_t nbits,
                        const char* desc);
 /* QuState::apply_unary_gate_all (qustate.rs:13, vectorstate.rs:180-189). */

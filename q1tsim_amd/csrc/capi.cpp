@@ -618,7 +618,7 @@ extern "C" int q1t_gvs_apply_gate(q1t_gvs* s, const double* mat, size_t nr_affec
     NEED(mat);
     NEED(bits);
     return guard([&] {
-        if (nr_affected > 10) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 10 qubits are not supported");
+        if (nr_affected > 12) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 12 qubits are not supported");
         const size_t d = size_t(1) << nr_affected;
         std::vector<cdouble> m = to_cd(mat, d * d);
         s->s.apply_gate(m.data(), nr_affected, bits, nbits, desc ? desc : "gate");
@@ -644,7 +644,7 @@ extern "C" int q1t_gvs_apply_conditional_gate(q1t_gvs* s, const uint8_t* control
     NEED(mat);
     NEED(bits);
     return guard([&] {
-        if (nr_affected > 10) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 10 qubits are not supported");
+        if (nr_affected > 12) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 12 qubits are not supported");
         const size_t d = size_t(1) << nr_affected;
         std::vector<cdouble> m = to_cd(mat, d * d);
         s->s.apply_conditional_gate(control, ncontrol, m.data(), nr_affected, bits, nbits, desc ? desc : "gate");

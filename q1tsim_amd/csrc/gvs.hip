@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_denseK(V* __restrict__ amp, uint64_t
     }
 }
 
-// Dense update for 6..10 target qubits: one workgroup per amplitude group,
+// Dense update for 6..12 target qubits: one workgroup per amplitude group,
 // the group staged in LDS, matrix rows read from global memory.
 template <typename V>
 __global__ void __launch_bounds__(kBlock) k_dense_big(V* __restrict__ amp, uint64_t col_stride, uint64_t ngroups,
@@ -763,7 +763,7 @@ void GpuVectorState::apply_gate(const cdouble* mat, size_t nr_affected, const si
 {
     if (nr_affected != nbits) throw err_invalid_nr_bits(nbits, nr_affected, desc);
     for (size_t k = 0; k < nbits; ++k) check_qbit(qbits[k]);
-    if (nbits > 10) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 10 qubits are not supported");
+    if (nbits > 12) throw Error(ERR_NOT_SUPPORTED, "Gates on more than 12 qubits are not supported");
     PendingGate pg;
     pg.op = analyze_gate(mat, nbits, qbits, nr_bits_);
     const size_t d = size_t(1) << nbits;
@@ -890,9 +890,14 @@ static void launch_op(const GateOp& op, V* amp, uint64_t col_stride, size_t ncol
                 else if (kt == 5)
                     hipLaunchKernelGGL((k_denseK<V, 5>), dim3(grid_for(nwork), ny), dim3(kBlock), 0, st, a, col_stride,
                                        nwork, pl, op.ctrl_mask, tg, dm);
-                else
+                else {
+                    // a group of 2^kt amplitudes in dynamic LDS: 64 KiB for 12 complex128 targets
+                    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dense_big<V>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(d * sizeof(V))) != hipSuccess)
+                        throw Error(ERR_HIP, "LDS attribute of the wide dense kernel");
                     hipLaunchKernelGGL(k_dense_big<V>, dim3(grid_for(nwork * kBlock), ny), dim3(kBlock), d * sizeof(V),
                                        st, a, col_stride, nwork, pl, op.ctrl_mask, tg, dm);
+                }
             }
             break;
         }

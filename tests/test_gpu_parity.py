@@ -448,3 +448,32 @@ def test_layered28_roundtrip():
     err = float(np.abs(col - e0).max())
     print("roundtrip n=28 max-abs: %.3e" % err)
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_wide_dense_gates(precision):
+    """Gates on 11 and 12 qubits known only by their matrix (the widest the
+    backend takes; a group of 2^12 complex128 amplitudes fills 64 KiB of LDS),
+    plain and with a control found from the matrix, against the dense
+    simulator."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    n = 14
+    a = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    a /= np.linalg.norm(a)
+    g = GpuVectorState.from_amplitudes(a, 1, precision)
+    ref = a.copy()
+    for k in (11, 12):
+        q, _ = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
+        bits = [int(b) for b in rng.permutation(n)[:k]]
+        g.apply_gate(q, bits)
+        ref = dense_sim.apply(ref, q, bits, n)
+    q, _ = np.linalg.qr(rng.normal(size=(1 << 11, 1 << 11)) + 1j * rng.normal(size=(1 << 11, 1 << 11)))
+    cq = np.eye(1 << 12, dtype=complex)
+    cq[1 << 11:, 1 << 11:] = q
+    cbits = [int(b) for b in rng.permutation(n)[:12]]
+    g.apply_gate(cq, cbits)
+    ref = dense_sim.apply(ref, cq, cbits, n)
+    close(g.column(0), ref, 1e-12 if precision == "c128" else 2e-5)
+    # rejected before the matrix is read (a stand-in of the right row count)
+    with pytest.raises(Exception, match="more than 12 qubits"):
+        g.apply_gate(np.zeros((1 << 13, 1), dtype=complex), list(range(13)))
