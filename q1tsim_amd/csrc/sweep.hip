@@ -38,14 +38,14 @@ namespace q1t {
                         std::string("HIP error in " #expr ": ") + hipGetErrorString(e_));          \
     } while (0)
 
-template <typename V, int M, int R, int SW, bool WIDE>
+template <typename V, int M, int R, int SW, int BLK>
 __global__ void __launch_bounds__(1 << (M - R))
 __attribute__((amdgpu_waves_per_eu(sizeof(V) == 8 && M == 12 ? (R == 3 ? 4 : 3) : 2)))
 k_sweep(V* __restrict__ amp, uint64_t col_stride, SweepArgs sa)
 {
     __shared__ V lds[1 << M];
     V* a = amp + uint64_t(blockIdx.y) * col_stride;
-    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) sweep_tile<V, M, R, SW, WIDE>(a, sa, tile, lds);
+    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) sweep_tile<V, M, R, SW, BLK>(a, sa, tile, lds);
 }
 
 // Sweeps of many states in one launch: item i covers flattened tiles
@@ -60,9 +60,11 @@ struct SweepItem {
     SweepArgs sa;
 };
 
-template <typename V, int M, int R, int SW, bool WIDE>
+// The complex64 12-bit tiles with 3 register bits and no fused blocks fit
+// 64 VGPRs (8 waves per SIMD, 4 workgroups per CU by LDS) without spilling.
+template <typename V, int M, int R, int SW, int BLK>
 __global__ void __launch_bounds__(1 << (M - R))
-__attribute__((amdgpu_waves_per_eu(2)))
+__attribute__((amdgpu_waves_per_eu(BLK == 0 && sizeof(V) == 8 && M == 12 && R == 3 ? 8 : 2)))
 k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_tiles, uint32_t chunk)
 {
     __shared__ V lds[1 << M];
@@ -84,14 +86,26 @@ k_sweep_batched(const SweepItem* __restrict__ items, int nitems, uint64_t total_
             const SweepItem& it = items[lo];
             const uint64_t lt = t - it.first_tile;
             V* a = static_cast<V*>(it.amp) + (lt >> it.log_ntiles) * it.col_stride;
-            sweep_tile<V, M, R, SW, WIDE>(a, it.sa, lt & ((uint64_t(1) << it.log_ntiles) - 1), lds);
+            sweep_tile<V, M, R, SW, BLK>(a, it.sa, lt & ((uint64_t(1) << it.log_ntiles) - 1), lds);
         }
     }
 }
 
+// One batched complex64 sweep round on the kernel holding the round's
+// widest block (blk as block_level).
+template <int M, int R, int SW>
+static void launch_batched_c64(int blk, unsigned gx, hipStream_t stream, const SweepItem* items, int nitems,
+                               uint64_t total_tiles, uint32_t chunk)
+{
+    const dim3 grid(gx), block(1u << (M - R));
+    if (blk == 2) hipLaunchKernelGGL((k_sweep_batched<float2, M, R, SW, 2>), grid, block, 0, stream, items, nitems, total_tiles, chunk);
+    else if (blk == 1) hipLaunchKernelGGL((k_sweep_batched<float2, M, R, SW, 1>), grid, block, 0, stream, items, nitems, total_tiles, chunk);
+    else hipLaunchKernelGGL((k_sweep_batched<float2, M, R, SW, 0>), grid, block, 0, stream, items, nitems, total_tiles, chunk);
+}
+
 template <typename V, int M, int R, int SW>
 static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase* dphases, int nphases,
-                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce, bool wide,
+                         const SwOp* dops, const float* dblocks, size_t cb, size_t ce, int blk,
                          hipFunction_t jit = nullptr)
 {
     const int n = int(st.nr_bits());
@@ -123,16 +137,16 @@ static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase*
                         2.0 * double(dim) * sizeof(V) * ny);
             continue;
         }
+        V* amp = static_cast<V*>(st.col_ptr(c0));
         if constexpr (sizeof(V) == 8) {
-            if (wide)
-                hipLaunchKernelGGL((k_sweep<V, M, R, SW, true>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
-                                   static_cast<V*>(st.col_ptr(c0)), dim, sa);
+            if (blk == 2)
+                hipLaunchKernelGGL((k_sweep<V, M, R, SW, 2>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(), amp, dim, sa);
+            else if (blk == 1)
+                hipLaunchKernelGGL((k_sweep<V, M, R, SW, 1>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(), amp, dim, sa);
             else
-                hipLaunchKernelGGL((k_sweep<V, M, R, SW, false>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
-                                   static_cast<V*>(st.col_ptr(c0)), dim, sa);
+                hipLaunchKernelGGL((k_sweep<V, M, R, SW, 0>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(), amp, dim, sa);
         } else {
-            hipLaunchKernelGGL((k_sweep<V, M, R, SW, false>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(),
-                               static_cast<V*>(st.col_ptr(c0)), dim, sa);
+            hipLaunchKernelGGL((k_sweep<V, M, R, SW, 0>), dim3(gx, ny), dim3(1 << (M - R)), 0, st.stream(), amp, dim, sa);
         }
         SW_HIP_OK(hipGetLastError());
         st.prof_end(ps, sizeof(V) == 16 ? "k_sweep_c128" : "k_sweep_c64", 2.0 * double(dim) * sizeof(V) * ny);
@@ -140,13 +154,14 @@ static int launch_sweep(GpuVectorState& st, const PlanStep& step, const SwPhase*
     return first;
 }
 
-// Whether a program holds a dense block wider than 4 qubits (launched on
-// the WIDE kernels).
-static bool has_wide_blocks(const std::vector<SwPhase>& phases)
+// The widest fused dense block of a program, as the kernels' BLK: 0 none,
+// 1 blocks of 3 and 4 qubits, 2 wider ones.
+static int block_level(const std::vector<SwPhase>& phases)
 {
+    int lv = 0;
     for (const SwPhase& ph : phases)
-        if (ph.kind == 1 && ph.reg_mask > 4) return true;
-    return false;
+        if (ph.kind == 1) lv = std::max(lv, ph.reg_mask > 4 ? 2 : 1);
+    return lv;
 }
 
 // One sweep's program: phases, ops and its own block table (block phases
@@ -425,7 +440,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             const SwPhase* dp = reinterpret_cast<const SwPhase*>(dprog + poff[s]);
             const SwOp* dop = reinterpret_cast<const SwOp*>(dprog + ooff[s]);
             const int np = int(progs[s].phases.size());
-            const bool wide = has_wide_blocks(progs[s].phases);
+            const int blk = block_level(progs[s].phases);
             int slot;
             // the compiled tile shapes (M local bits, R register bits): see tile_shape_supported()
             const hipFunction_t jf = jitfn[s];
@@ -435,15 +450,15 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
             const int shape = g.M * 16 + g.R;
             if (st.precision() == PREC_C128) {
                 switch (shape) {
-                case 12 * 16 + 3: slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf); break;
-                case 12 * 16 + 4: slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, false, jf); break;
+                case 12 * 16 + 3: slot = launch_sweep<double2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, 0, jf); break;
+                case 12 * 16 + 4: slot = launch_sweep<double2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, 0, jf); break;
                 default: throw Error(ERR_INVALID_ARGUMENT, "No sweep kernel for this complex128 tile shape");
                 }
             } else {
                 switch (shape) {
-                case 13 * 16 + 4: slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
-                case 12 * 16 + 3: slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
-                case 12 * 16 + 4: slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, wide, jf); break;
+                case 13 * 16 + 4: slot = launch_sweep<float2, 13, 4, 5>(st, step, dp, np, dop, dblocks, col_begin, col_end, blk, jf); break;
+                case 12 * 16 + 3: slot = launch_sweep<float2, 12, 3, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, blk, jf); break;
+                case 12 * 16 + 4: slot = launch_sweep<float2, 12, 4, 4>(st, step, dp, np, dop, dblocks, col_begin, col_end, blk, jf); break;
                 default: throw Error(ERR_INVALID_ARGUMENT, "No sweep kernel for this complex64 tile shape");
                 }
             }
@@ -488,7 +503,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
                 put(poff[s]);
                 put(ooff[s]);
                 put(progs[s].phases.size());
-                put(has_wide_blocks(progs[s].phases) ? 1 : 0);
+                put(uint64_t(block_level(progs[s].phases)));
                 continue;
             }
             for (size_t i : steps[s].gates) {
@@ -805,7 +820,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         std::vector<double2> svals;
         std::vector<size_t> poff, ooff;
         std::vector<int> nph;
-        std::vector<char> wide;   // per step: a block wider than 4 qubits
+        std::vector<char> blk;    // per step: block_level of its program
         bool small = false;
         int sop_begin = 0, sop_end = 0;
         RunStats st;
@@ -895,10 +910,10 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         p.poff.assign(p.steps.size(), 0);
         p.ooff.assign(p.steps.size(), 0);
         p.nph.assign(p.steps.size(), 0);
-        p.wide.assign(p.steps.size(), 0);
+        p.blk.assign(p.steps.size(), 0);
         for (size_t s = 0; s < p.steps.size(); ++s) {
             if (!p.steps[s].sweep) continue;
-            p.wide[s] = has_wide_blocks(p.ph[s]);
+            p.blk[s] = char(block_level(p.ph[s]));
             p.poff[s] = off;
             off = up(off + p.ph[s].size() * sizeof(SwPhase));
             p.ooff[s] = off;
@@ -922,7 +937,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
     // round_first[r][i]
     std::vector<std::pair<size_t, size_t>> round_items(max_steps);   // (first item, count)
     std::vector<uint64_t> round_tiles(max_steps, 0);
-    std::vector<char> round_wide(max_steps, 0);
+    std::vector<char> round_blk(max_steps, 0);
     std::vector<std::vector<uint64_t>> item_idx(max_steps), item_tile(max_steps);
     size_t wi = 0;
     for (size_t r = 0; r < max_steps; ++r) {
@@ -933,7 +948,7 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         for (size_t i = 0; i < live.size(); ++i) {
             const Plan& p = plans[i];
             if (p.small || r >= p.steps.size() || !p.steps[r].sweep) continue;
-            round_wide[r] |= p.wide[r];
+            round_blk[r] = std::max(round_blk[r], p.blk[r]);
             item_idx[r][i] = wi++;
             item_tile[r][i] = tiles;
             tiles += uint64_t(live[i]->nr_cols()) << (live[i]->nr_bits() - size_t(g.M));
@@ -1050,16 +1065,14 @@ void run_planned_batch(const std::vector<GpuVectorState*>& states, hipStream_t s
         const uint64_t nchunks = (round_tiles[r] + tile_chunk - 1) / tile_chunk;
         const unsigned gx = unsigned(std::min<uint64_t>(nchunks, 4096));
         if (prec == PREC_C128) {
-            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-        } else if (round_wide[r]) {
-            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, true>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, true>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<double2, 12, 3, 4, 0>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            else hipLaunchKernelGGL((k_sweep_batched<double2, 12, 4, 4, 0>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+        } else if (g.M == 13) {
+            launch_batched_c64<13, 4, 5>(round_blk[r], gx, stream, ditems, nit, round_tiles[r], tile_chunk);
+        } else if (g.R == 3) {
+            launch_batched_c64<12, 3, 4>(round_blk[r], gx, stream, ditems, nit, round_tiles[r], tile_chunk);
         } else {
-            if (g.M == 13) hipLaunchKernelGGL((k_sweep_batched<float2, 13, 4, 5, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-            else if (g.R == 3) hipLaunchKernelGGL((k_sweep_batched<float2, 12, 3, 4, false>), dim3(gx), dim3(512), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
-            else hipLaunchKernelGGL((k_sweep_batched<float2, 12, 4, 4, false>), dim3(gx), dim3(256), 0, stream, ditems, nit, round_tiles[r], tile_chunk);
+            launch_batched_c64<12, 4, 4>(round_blk[r], gx, stream, ditems, nit, round_tiles[r], tile_chunk);
         }
         SW_HIP_OK(hipGetLastError());
         agg.launches += 1;

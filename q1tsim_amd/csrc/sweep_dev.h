@@ -52,6 +52,19 @@ template <int SW, int M> __device__ __forceinline__ int swz(int l)
     return l ^ (f & ((1 << SW) - 1));
 }
 
+// LDS slot of the thread's tile element tid + k * 2^(M-R) is
+// swz(tid) ^ swz(k * 2^(M-R)) (swz is XOR-linear; the second term is a
+// constant).  The lane index passes through an empty asm statement, so
+// swz(tid) is recomputed at each tile load and store (a few ALU ops) instead
+// of the compiler hoisting all 2^R slots out of the tile loop, where they
+// would hold 2^R registers for the kernel's lifetime and lower its occupancy.
+template <int SW, int M> __device__ __forceinline__ int tile_slot_base()
+{
+    int t = int(threadIdx.x);
+    asm volatile("" : "+v"(t));
+    return swz<SW, M>(t);
+}
+
 // c ? a : b per component (a conditional on whole vector structs selects
 // between their addresses, which keeps register arrays out of registers).
 template <typename V> __device__ __forceinline__ V sel(bool c, V a, V b)
@@ -475,10 +488,11 @@ __device__ __forceinline__ V tile_factor(const SwOp* ops, uint32_t begin, uint32
     return f;
 }
 
-// The program's phases on a tile already in LDS, then LDS -> HBM.  WIDE
-// kernels also hold the 5- and 6-qubit dense blocks; the others only 3 and 4,
-// which keeps the wide blocks' registers out of the common kernels.
-template <typename V, int M, int R, int SW, bool WIDE>
+// The program's phases on a tile already in LDS, then LDS -> HBM.  BLK is
+// the widest fused dense block the kernel holds: 0 none (programs without
+// blocks: the MFMA code's registers would halve the kernel's occupancy),
+// 1 the 3- and 4-qubit blocks, 2 also the 5- and 6-qubit ones.
+template <typename V, int M, int R, int SW, int BLK>
 __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& sa, uint64_t base, V* lds,
                                             const TileAddr<M, R>& ta)
 {
@@ -490,13 +504,13 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
             // the program is read through the constant address space: scalar
             // loads, whatever else the kernel writes (including the DMA)
             const SwPhase ph = load_uniform(sa.phases, uint32_t(p));
-            if constexpr (sizeof(V) == 8) {
+            if constexpr (sizeof(V) == 8 && BLK > 0) {
                 if (ph.kind == 1) {
                     // fused dense block: K = reg_mask, matrix at blocks + op_begin
                     const float* A = sa.blocks + ph.op_begin;
-                    // the planner emits K = 3 .. 6, and K > 4 only to WIDE kernels
+                    // the planner emits K = 3 .. 6, and K > 4 only to BLK 2 kernels
                     float2* l2 = reinterpret_cast<float2*>(lds);
-                    if constexpr (WIDE) {
+                    if constexpr (BLK == 2) {
                         switch (ph.reg_mask) {
                         case 3: block_mfma<3, M, SW>(l2, A, ph, tid); break;
                         case 4: block_mfma<4, M, SW>(l2, A, ph, tid); break;
@@ -579,24 +593,28 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
             __syncthreads();
         }
 
+        {
+            const int s0 = tile_slot_base<SW, M>();
 #pragma unroll
-        for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];
+            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[s0 ^ swz<SW, M>(k * NT)];
+        }
         __syncthreads();
     }
 }
 
 // One tile of a sweep: HBM -> LDS, the program's phases, LDS -> HBM.
-template <typename V, int M, int R, int SW, bool WIDE>
+template <typename V, int M, int R, int SW, int BLK>
 __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& sa, uint64_t tile, V* lds)
 {
     constexpr int NT = 1 << (M - R), NR = 1 << R;
     TileAddr<M, R> ta;
     tile_addr<M, R>(sa, ta);
     const uint64_t base = deposit(tile, sa.tset);
+    const int s0 = tile_slot_base<SW, M>();
 #pragma unroll
-    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(int(threadIdx.x) + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];
+    for (int k = 0; k < NR; ++k) lds[s0 ^ swz<SW, M>(k * NT)] = a[base | ta.g_tid | ta.g_k[k]];
     __syncthreads();
-    tile_phases<V, M, R, SW, WIDE>(a, sa, base, lds, ta);
+    tile_phases<V, M, R, SW, BLK>(a, sa, base, lds, ta);
 }
 
 }  // namespace q1t
