@@ -18,12 +18,7 @@
 namespace q1t {
 
 // LDS slot of local element l (the device swz<SW, M>).
-static int swz_host(int SW, int M, int l)
-{
-    int f = 0;
-    for (int s = SW; s < M; s += SW) f ^= l >> s;
-    return l ^ (f & ((1 << SW) - 1));
-}
+static int swz_host(int SW, int M, int l) { return swizzle_slot(SW, M, l); }
 
 static std::string hex64(uint64_t v)
 {

@@ -287,6 +287,13 @@ SwPhase make_phase(uint32_t kind, uint32_t regs, int M, uint32_t op_begin)
         if ((regs >> b) & 1) ph.rpos[nr++] = uint8_t(b);
         else ph.tbit[nt++] = uint8_t(b);
     }
+    if (kind == 0 && nr <= 4)
+        for (int k = 0; k < (1 << nr); ++k) {
+            int l = 0;
+            for (int b = 0; b < nr; ++b)
+                if ((k >> b) & 1) l |= 1 << ph.rpos[b];
+            ph.rslot[k] = uint32_t(swizzle_slot(sweep_swizzle_bits(M), M, l));
+        }
     return ph;
 }
 

@@ -358,7 +358,7 @@ std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int
         for (SwPhase& ph : p.phases)
             if (ph.kind == 1) ph.op_begin += uint32_t(nblk_floats);
         nblk_floats += p.blocks.size();
-        out.push_back(jit_sweep_source(p.phases, p.ops, prec == PREC_C64, g.M, g.R, g.M == 13 ? 5 : 4, prefetch));
+        out.push_back(jit_sweep_source(p.phases, p.ops, prec == PREC_C64, g.M, g.R, sweep_swizzle_bits(g.M), prefetch));
     }
     return out;
 }
@@ -414,7 +414,7 @@ void run_planned(GpuVectorState& st, const std::vector<PendingGate>& queue, size
         for (size_t s = 0; s < steps.size(); ++s)
             if (steps[s].sweep) {
                 src[s] = jit_sweep_source(progs[s].phases, progs[s].ops, st.precision() == PREC_C64, g.M, g.R,
-                                          g.M == 13 ? 5 : 4, st.jit_prefetch());
+                                          sweep_swizzle_bits(g.M), st.jit_prefetch());
                 all.push_back(src[s]);
             }
         JitCache& jc = JitCache::get();

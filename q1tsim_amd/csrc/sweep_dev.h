@@ -42,14 +42,26 @@ template <> __device__ __forceinline__ float2 opm<float2>(const SwOp& op, int k)
     return mkc<float2>(op.mf[2 * k], op.mf[2 * k + 1]);
 }
 
+// Makes every field the op bodies read live at this point, so the op's
+// scalar loads issue together and the op waits once for them (else the mask
+// test's load is waited for first and the rest loaded after the branch).
+template <typename V> __device__ __forceinline__ void pin_op(const SwOp& op)
+{
+    asm volatile("" ::"s"(op.type), "s"(op.reg), "s"(op.dreg), "s"(op.dtid), "s"(op.dglobal), "s"(op.cr),
+                 "s"(op.ct), "s"(op.aux), "s"(op.co));
+    if constexpr (sizeof(V) == 8)
+        asm volatile("" ::"s"(op.mf[0]), "s"(op.mf[1]), "s"(op.mf[2]), "s"(op.mf[3]), "s"(op.mf[4]), "s"(op.mf[5]),
+                     "s"(op.mf[6]), "s"(op.mf[7]));
+    else
+        asm volatile("" ::"s"(op.m[0]), "s"(op.m[1]), "s"(op.m[2]), "s"(op.m[3]), "s"(op.m[4]), "s"(op.m[5]),
+                     "s"(op.m[6]), "s"(op.m[7]));
+}
+
 // LDS slot of local element l: XOR-fold higher bits into the low SW bits so
 // strided phase exchanges spread over the banks (an involution).
 template <int SW, int M> __device__ __forceinline__ int swz(int l)
 {
-    int f = 0;
-#pragma unroll
-    for (int s = SW; s < M; s += SW) f ^= l >> s;
-    return l ^ (f & ((1 << SW) - 1));
+    return swizzle_slot(SW, M, l);
 }
 
 // LDS slot of the thread's tile element tid + k * 2^(M-R) is
@@ -538,20 +550,16 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
             // have disjoint bits: slot k's LDS index = swz(tpart) ^ swz(rp[k]),
             // the second term wave-uniform
             const int st = swz<SW, M>(int(tpart));
+            static_assert(SW == sweep_swizzle_bits(M), "the planner's register slot terms use this swizzle");
             int srp[NR];
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                uint32_t l = 0;
-#pragma unroll
-                for (int b = 0; b < R; ++b)
-                    if ((k >> b) & 1) l |= 1u << ph.rpos[b];
-                srp[k] = swz<SW, M>(int(l));
-            }
+            for (int k = 0; k < NR; ++k) srp[k] = int(ph.rslot[k]);
             V v[NR];
 #pragma unroll
             for (int k = 0; k < NR; ++k) v[k] = lds[st ^ srp[k]];
             for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
                 const SwOp op = load_uniform(sa.ops, oi);
+                pin_op<V>(op);
                 if ((base & op.co) != op.co) continue;
                 const uint32_t cr = op.cr, ct = op.ct;
                 const bool tok = (uint32_t(tid) & ct) == ct;

@@ -63,6 +63,23 @@ struct SwPhase {
     uint32_t wmap;       // 0: identity (plain write-back)
     uint32_t wxor;
     uint16_t wcol[16];
+    // register phases: LDS slot term of register slot k, swz of the local
+    // index made of k's bits at the register positions (the slot of a
+    // thread's element k is swz(thread part) ^ rslot[k]); set by the planner
+    // for the kernel's swizzle (sweep_swizzle_bits)
+    uint32_t rslot[16];
 };
+
+// LDS swizzle of local element l in a 2^M tile: XOR-fold of the bits above
+// SW into the low SW bits (an involution, XOR-linear in l).
+constexpr int swizzle_slot(int SW, int M, int l)
+{
+    int f = 0;
+    for (int s = SW; s < M; s += SW) f ^= l >> s;
+    return l ^ (f & ((1 << SW) - 1));
+}
+
+// Swizzle width of the sweep kernels for 2^M tiles (12 -> 4, 13 -> 5).
+constexpr int sweep_swizzle_bits(int M) { return M - 8; }
 
 }  // namespace q1t
