@@ -525,26 +525,43 @@ std::vector<size_t> one_reg_phase(const std::vector<PendingGate>& q, const std::
     }
     std::vector<size_t> chosen, rest;
     phase_absorb(q, pending, lm, regs, &chosen, &rest);
+    // Permutation gates taken as register ops (X, CX and SWAP with their
+    // target in a register) that no later register op of the phase touches
+    // commute with every later op: they join the write-back map (composed
+    // in queue order after the ops) and cost no op.
+    std::vector<char> as_op(chosen.size(), 0);
+    {
+        uint64_t later = 0;   // bits of the later gates that stay register ops
+        for (size_t x = chosen.size(); x-- > 0;) {
+            const GateOp& op = q[chosen[x]].op;
+            if (phase_role(op, lm, regs) != ROLE_OP) continue;
+            const uint64_t bits = touched_bits(op);
+            if (writeback_gate(op, lm) && !(bits & later)) continue;
+            as_op[x] = 1;
+            later |= bits;
+        }
+    }
     SwPhase ph = make_phase(0, regs, M, uint32_t(ops.size()));
     // the write-back map, composed in queue order: f(l) = XOR_j l_j col[j] ^ c
     uint32_t col[16], cst = 0;
     for (int j = 0; j < 16; ++j) col[j] = 1u << j;
-    auto is_factor = [&](size_t i) {
-        return tf && phase_role(q[i].op, lm, regs) == ROLE_OP && tile_scalar(make_op(q[i].op, lm, regs));
+    auto is_factor = [&](size_t x) {
+        return tf && as_op[x] && tile_scalar(make_op(q[chosen[x]].op, lm, regs));
     };
     if (order) {   // execution order: the register ops, then the write-back gates
-        for (size_t i : chosen)
-            if (phase_role(q[i].op, lm, regs) == ROLE_OP && !is_factor(i)) order->push_back(i);
-        for (size_t i : chosen)
-            if (phase_role(q[i].op, lm, regs) != ROLE_OP) order->push_back(i);
+        for (size_t x = 0; x < chosen.size(); ++x)
+            if (as_op[x] && !is_factor(x)) order->push_back(chosen[x]);
+        for (size_t x = 0; x < chosen.size(); ++x)
+            if (!as_op[x]) order->push_back(chosen[x]);
     }
     auto apply = [&](auto&& g) {   // f := g o f for a map g on local indices
         for (int j = 0; j < M; ++j) col[j] = g(col[j], false);
         cst = g(cst, true);
     };
-    for (size_t i : chosen) {
+    for (size_t ci = 0; ci < chosen.size(); ++ci) {
+        const size_t i = chosen[ci];
         const GateOp& op = q[i].op;
-        if (phase_role(op, lm, regs) == ROLE_OP) {
+        if (as_op[ci]) {
             const SwOp so = make_op(op, lm, regs);
             if (tf && tile_scalar(so)) {
                 tf->ops.push_back(so);
