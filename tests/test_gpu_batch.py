@@ -86,7 +86,9 @@ def test_batch_end_states_c128():
             assert np.max(np.abs(a - b)) <= 1e-12, (n, s, col)
 
 
-@pytest.mark.parametrize("fusion", [0, 4])
+# fusion 0, 4 and 6: the batched sweep kernels without MFMA code, with the
+# 3-4-qubit blocks and with the 5-6-qubit blocks
+@pytest.mark.parametrize("fusion", [0, 4, 6])
 def test_batch_c64(fusion):
     wl = [(n, nc, prog[:-1], s) for (n, nc, prog, s) in WL]
     _, circs = run_batch(wl, "c64", fusion=fusion, keep=True)
