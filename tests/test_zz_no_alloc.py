@@ -36,7 +36,9 @@ def test_noalloc_marginal_matches_marginal(precision):
             got = st.marginal_w0_noalloc(q)
             assert got.shape == exp.shape
             assert np.array_equal(got, exp), (st.nr_bits, q, got, exp)
-            assert np.all((got >= 0) & (got <= 1 + 1e-6))
+            # a probability, up to the state's norm drift (FP32 amplitudes
+            # drift by ~1e-7 per gate)
+            assert np.all((got >= 0) & (got <= 1 + (1e-5 if precision == "c64" else 1e-9)))
     big.close()
     small.close()
 
