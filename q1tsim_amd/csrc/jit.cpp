@@ -54,6 +54,7 @@ static void emit_register_phase(std::ostringstream& o, const SwPhase& ph, const 
         srp[k] = swz_host(SW, M, int(l));
         o << "        " << (noops ? "const V " : "") << vname << k << vend << " = lds[st ^ " << srp[k] << "];\n";
     }
+    o << "        __builtin_amdgcn_s_setprio(0);\n";
     for (uint32_t oi = ph.op_begin; oi < ph.op_end; ++oi) {
         const SwOp& op = ops[oi];
         o << "        {\n";
@@ -103,6 +104,9 @@ static void emit_register_phase(std::ostringstream& o, const SwPhase& ph, const 
         if (op.co) o << "            }\n";
         o << "        }\n";
     }
+    // the phase's LDS exchange (write-back, barrier, the next phase's reads)
+    // at priority 1: between the tile I/O (2) and the arithmetic (0)
+    o << "        __builtin_amdgcn_s_setprio(1);\n";
     if (ph.wmap == 0) {
         for (int k = 0; k < NR; ++k) o << "        lds[st ^ " << srp[k] << "] = " << vname << k << vend << ";\n";
     } else {
@@ -169,8 +173,9 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
     // loads of the next) runs at a raised priority, the phases at the base
     // one: with two workgroups per CU, the one at its tile boundary gets its
     // memory instructions issued ahead of the other's arithmetic, which keeps
-    // HBM busier (30-qubit layered circuit: c128 135 -> 128 ms, c64 83 -> 80 ms;
-    // the interpreter kernels measured no gain and are left as they are).
+    // HBM busier (30-qubit layered circuit: c128 135 -> 128 ms, c64 83 -> 80 ms,
+    // and another 1 % with the LDS exchanges at an intermediate priority; the
+    // interpreter kernels measured no gain and are left as they are).
     o << "    __builtin_amdgcn_s_setprio(2);\n";
     o << "    for (uint64_t tile = blockIdx.x; tile < sa.ntiles; tile += gridDim.x) {\n";
     // The thread index and the table pointers are made opaque per tile:
@@ -190,7 +195,7 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
         for (int k = 0; k < NR; ++k) o << "        pf" << k << " = a[nb | ta.g_tid | ta.g_k[" << k << "]];\n";
         o << "    }\n";
     } else {
-        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = a[base | ta.g_tid | ta.g_k[k]];\n";
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) lds[swz<SW, M>(tid + k * NT)] = load_stream(a + (base | ta.g_tid | ta.g_k[k]));\n";
         o << "    __syncthreads();\n";
     }
     o << "    __builtin_amdgcn_s_setprio(0);\n";
@@ -220,9 +225,9 @@ std::string jit_sweep_source(const std::vector<SwPhase>& phases, const std::vect
     }
     o << "    __builtin_amdgcn_s_setprio(2);\n";
     if (scaled)
-        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = cmul(fac, lds[swz<SW, M>(tid + k * NT)]);\n";
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) store_stream(a + (base | ta.g_tid | ta.g_k[k]), cmul(fac, lds[swz<SW, M>(tid + k * NT)]));\n";
     else
-        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[swz<SW, M>(tid + k * NT)];\n";
+        o << "#pragma unroll\n    for (int k = 0; k < NR; ++k) store_stream(a + (base | ta.g_tid | ta.g_k[k]), lds[swz<SW, M>(tid + k * NT)]);\n";
     o << "    __syncthreads();\n    }\n}\n";
     return o.str();
 }

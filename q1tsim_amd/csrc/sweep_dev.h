@@ -77,6 +77,25 @@ template <int SW, int M> __device__ __forceinline__ int tile_slot_base()
     return swz<SW, M>(t);
 }
 
+// Tile I/O with the non-temporal hint: every amplitude of a sweep is read
+// once and written once, so it need not stay in the caches.
+template <typename V> __device__ __forceinline__ V load_stream(const V* p)
+{
+    typedef typename RealOf<V>::type Rl;
+    typedef Rl Rl2 __attribute__((ext_vector_type(2)));
+    const Rl2 x = __builtin_nontemporal_load(reinterpret_cast<const Rl2*>(p));
+    return mkc<V>(x.x, x.y);
+}
+template <typename V> __device__ __forceinline__ void store_stream(V* p, V v)
+{
+    typedef typename RealOf<V>::type Rl;
+    typedef Rl Rl2 __attribute__((ext_vector_type(2)));
+    Rl2 x;
+    x.x = v.x;
+    x.y = v.y;
+    __builtin_nontemporal_store(x, reinterpret_cast<Rl2*>(p));
+}
+
 // c ? a : b per component (a conditional on whole vector structs selects
 // between their addresses, which keeps register arrays out of registers).
 template <typename V> __device__ __forceinline__ V sel(bool c, V a, V b)
