@@ -623,7 +623,7 @@ __device__ __forceinline__ void tile_phases(V* __restrict__ a, const SweepArgs& 
         {
             const int s0 = tile_slot_base<SW, M>();
 #pragma unroll
-            for (int k = 0; k < NR; ++k) a[base | ta.g_tid | ta.g_k[k]] = lds[s0 ^ swz<SW, M>(k * NT)];
+            for (int k = 0; k < NR; ++k) store_stream(a + (base | ta.g_tid | ta.g_k[k]), lds[s0 ^ swz<SW, M>(k * NT)]);
         }
         __syncthreads();
     }
@@ -639,7 +639,7 @@ __device__ __forceinline__ void sweep_tile(V* __restrict__ a, const SweepArgs& s
     const uint64_t base = deposit(tile, sa.tset);
     const int s0 = tile_slot_base<SW, M>();
 #pragma unroll
-    for (int k = 0; k < NR; ++k) lds[s0 ^ swz<SW, M>(k * NT)] = a[base | ta.g_tid | ta.g_k[k]];
+    for (int k = 0; k < NR; ++k) lds[s0 ^ swz<SW, M>(k * NT)] = load_stream(a + (base | ta.g_tid | ta.g_k[k]));
     __syncthreads();
     tile_phases<V, M, R, SW, BLK>(a, sa, base, lds, ta);
 }
