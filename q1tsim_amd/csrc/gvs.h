@@ -100,6 +100,9 @@ public:
 private:
     static size_t default_cap();
     void evict_to(size_t limit);   // caller holds mu_
+    // frees cached blocks until `need` bytes are freed (other size classes
+    // than `cls` first, oldest first); returns the bytes freed
+    size_t evict_for(size_t need, size_t cls);
     struct Entry {
         void* p;
         size_t bytes;     // the size class

@@ -375,6 +375,31 @@ void DeviceCache::evict_to(size_t limit)
     }
 }
 
+size_t DeviceCache::evict_for(size_t need, size_t cls)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t freed = 0;
+    // the oldest blocks of other size classes first, then of `cls` itself
+    for (int pass = 0; pass < 2 && freed < need; ++pass) {
+        while (freed < need) {
+            auto oldest = free_.end();
+            for (auto it = free_.begin(); it != free_.end(); ++it) {
+                if (it->second.empty() || (pass == 0 && it->first == cls)) continue;
+                if (oldest == free_.end() || it->second.front().seq < oldest->second.front().seq) oldest = it;
+            }
+            if (oldest == free_.end()) break;
+            Entry e = oldest->second.front();
+            oldest->second.pop_front();
+            if (oldest->second.empty()) free_.erase(oldest);
+            drop_entry_event(e.ev, true);
+            HIP_OK(hipFree(e.p));
+            cached_ -= e.bytes;
+            freed += e.bytes;
+        }
+    }
+    return freed;
+}
+
 void DeviceCache::set_cap(size_t bytes)
 {
     std::lock_guard<std::mutex> lk(mu_);
@@ -427,9 +452,11 @@ void* DeviceCache::alloc(size_t bytes, hipStream_t s)
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, cls);
-    if (e == hipErrorOutOfMemory) {
+    while (e == hipErrorOutOfMemory) {
         (void)hipGetLastError();
-        trim();
+        // free just enough cached blocks and retry: emptying the whole cache
+        // would make every later request of this pass miss and allocate anew
+        if (evict_for(cls, cls) == 0) break;
         e = hipMalloc(&p, cls);
     }
     if (e != hipSuccess) {
