@@ -128,7 +128,7 @@ SIGNATURES = {
     "q1t_gvs_nr_bits": (_sz, [_vp]),
     "q1t_gvs_nr_cols": (_sz, [_vp]),
     "q1t_gvs_counts": (_i32, [_vp, _ps, _sz]),
-    "q1t_gvs_apply_gate": (_i32, [_vp, _pd, _sz, _ps, _sz, _cs]),
+    "q1t_gvs_apply_gate": (_i32, [_vp, _vp, _sz, _ps, _sz, _cs]),
     "q1t_gvs_apply_unary_gate_all": (_i32, [_vp, _pd, _cs]),
     "q1t_gvs_apply_conditional_gate": (_i32, [_vp, _pu8, _sz, _pd, _sz, _ps, _sz, _cs]),
     "q1t_gvs_marginal_w0": (_i32, [_vp, _sz, _pd, _sz]),

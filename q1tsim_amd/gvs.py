@@ -42,6 +42,41 @@ def gate_matrix(name, params=()):
     return m
 
 
+# Small matrices passed to apply_gate again and again (gate_matrix hands out
+# the same cached arrays) keep their address and size here, looked up by
+# identity: numpy's ctypes interface costs a few microseconds a call, about
+# half of the per-gate host time of a 900-gate circuit.  The entries hold
+# their arrays, so an address stays valid; the C side copies the matrix at
+# each call, so later changes to an array's values are seen.
+_arg_cache = {}
+_bits_cache = {}
+
+
+def _matrix_arg(matrix):
+    """(address, qubit count, object to keep alive during the call)."""
+    hit = _arg_cache.get(id(matrix))
+    if hit is not None and hit[0] is matrix:
+        return hit[1], hit[2], None
+    m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
+    addr, k = m.ctypes.data, m.shape[0].bit_length() - 1
+    if m is matrix and m.shape[0] <= 16:
+        if len(_arg_cache) >= 4096:
+            _arg_cache.clear()
+        _arg_cache[id(matrix)] = (matrix, addr, k)
+    return addr, k, m
+
+
+def _bits_arg(bits):
+    key = tuple(int(b) for b in bits)
+    arr = _bits_cache.get(key)
+    if arr is None:
+        arr = sizes(key)
+        if len(_bits_cache) >= 4096:
+            _bits_cache.clear()
+        _bits_cache[key] = arr
+    return arr
+
+
 def copy_bandwidth(nbytes=8 << 30, reps=5):
     """Device-to-device copy of `nbytes`, best of `reps` (HIP events):
     (ms, GB/s counting the bytes read and written)."""
@@ -194,9 +229,9 @@ class GpuVectorState:
         return np.stack([self.column(c) for c in range(self.ncols)], axis=1)
 
     def apply_gate(self, matrix, bits, desc="gate"):
-        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
-        k = int(np.log2(m.shape[0]))
-        check(lib().q1t_gvs_apply_gate(self.h, _dptr(m), k, sizes(bits), len(bits), desc.encode()))
+        addr, k, keep = _matrix_arg(matrix)
+        check(lib().q1t_gvs_apply_gate(self.h, addr, k, _bits_arg(bits), len(bits), desc.encode()))
+        del keep
 
     def apply_unary_gate_all(self, matrix, desc="gate"):
         m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128))
