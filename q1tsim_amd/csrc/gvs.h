@@ -357,6 +357,15 @@ private:
     bool geom_set_ = false;               // set_geometry was called
     bool graph_on_ = false;
     std::vector<FlushGraph> fgraphs_;
+
+public:
+    // the last per-gate plan of this state and its key (sweep.hip)
+    std::vector<uint64_t>& plan_cache_key() { return plan_key_; }
+    std::shared_ptr<void>& plan_cache_steps() { return plan_steps_; }
+
+private:
+    std::vector<uint64_t> plan_key_;
+    std::shared_ptr<void> plan_steps_;
     uint64_t graph_clock_ = 0, graph_captures_ = 0, graph_replays_ = 0;
 
     bool profiling_ = false;

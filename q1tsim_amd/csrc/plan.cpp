@@ -160,6 +160,25 @@ size_t absorb_fixed(const std::vector<GateBits>& gb, const std::vector<size_t>& 
 
 }  // namespace
 
+std::vector<uint64_t> plan_key(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k, bool search)
+{
+    std::vector<uint64_t> k;
+    k.reserve(4 * q.size() + 4);
+    k.push_back(uint64_t(n));
+    k.push_back(uint64_t(g.M) | uint64_t(g.R) << 8 | uint64_t(g.L) << 16);
+    k.push_back(uint64_t(fuse_k));
+    k.push_back(search ? 1 : 0);
+    for (const PendingGate& pg : q) {
+        uint64_t need = 0;
+        const bool ok = tile_need(pg.op, fuse_k, &need);
+        k.push_back(touched_bits(pg.op));
+        k.push_back(nondiag_bits(pg.op));
+        k.push_back(need);
+        k.push_back(uint64_t(pg.op.kind) | uint64_t(ok) << 8 | uint64_t(is_diagonal(pg.op)) << 9);
+    }
+    return k;
+}
+
 std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k, bool search)
 {
     std::vector<PlanStep> steps;

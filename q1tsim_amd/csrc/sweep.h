@@ -35,6 +35,11 @@ bool sweepable(const GateOp& op, std::vector<int>* local_targets);
 std::vector<PlanStep> plan_gates(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k,
                                  bool search = true);
 
+// Everything plan_gates reads of the queue and its settings: equal keys give
+// equal plans (a flush repeating the previous flush's gate structure, with
+// any angles, reuses its steps).
+std::vector<uint64_t> plan_key(const std::vector<PendingGate>& q, int n, const TileGeom& g, int fuse_k, bool search);
+
 // Register phases of one sweep (per-gate arithmetic).  `order`, if given,
 // receives the queue indices of the sweep's gates in the order the kernel
 // applies them (per phase: its register ops, then its write-back gates).

@@ -195,6 +195,12 @@ static double single_gate_us(double state_bytes) { return 2.0 * state_bytes / 4.
 // blocks, blocks of the configured width when it is 5 or 6} -- as ranked by
 // the learned cost model (PLAN_LEARNED), or a seeded random choice among the
 // same candidates (PLAN_EXPLORE).
+// A state's last plan (choose_plan): its steps and each sweep's program kind.
+struct PlanCacheEntry {
+    std::vector<PlanStep> steps;
+    std::vector<int> kind;
+};
+
 static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queue, int fusion_k, double state_bytes,
                         double tiles, std::vector<PlanStep>& steps, std::vector<SweepProgram>& progs)
 {
@@ -210,8 +216,37 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
         return out;
     };
     const bool fused = st.precision() == PREC_C64 && fusion_k > 0;
+    // A flush with the previous flush's gate structure (a repeated circuit,
+    // new angles allowed) takes that flush's steps and per-sweep program
+    // kinds and builds only those programs: the tile searches (and the
+    // learned planner's candidates) are most of the planning time.  Kept for
+    // states the tile search runs on.
+    std::vector<uint64_t> key;
+    auto cache_hit = [&](std::vector<int>& kind) {
+        const std::shared_ptr<void>& cached = st.plan_cache_steps();
+        if (!cached || key != st.plan_cache_key()) return false;
+        const PlanCacheEntry& e = *static_cast<const PlanCacheEntry*>(cached.get());
+        steps = e.steps;
+        kind = e.kind;
+        return true;
+    };
+    auto cache_store = [&](const std::vector<int>& kind) {
+        st.plan_cache_key() = key;
+        st.plan_cache_steps() = std::shared_ptr<void>(new PlanCacheEntry{steps, kind},
+                                                      [](void* p) { delete static_cast<PlanCacheEntry*>(p); });
+    };
     if (!fused) {
-        steps = plan_gates(queue, n, g, 0, search);
+        if (search) {
+            key = plan_key(queue, n, g, 0, search);
+            key.insert(key.begin(), 1);   // the per-gate planner
+            std::vector<int> kind;
+            if (!cache_hit(kind)) {
+                steps = plan_gates(queue, n, g, 0, search);
+                cache_store(std::vector<int>(steps.size(), 0));
+            }
+        } else {
+            steps = plan_gates(queue, n, g, 0, search);
+        }
         progs = build_all(steps, 0);
         return;
     }
@@ -220,9 +255,36 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     CostModel* cm = st.cost_model();
     const int mode = st.plan_mode();
     if (mode == PLAN_GREEDY || (mode == PLAN_LEARNED && (!cm || !cm->trained()))) {
-        steps = plan_gates(queue, n, g, Kg, search);
+        if (search) {
+            key = plan_key(queue, n, g, Kg, search);
+            key.insert(key.begin(), 3);   // greedy fusion
+            std::vector<int> kind;
+            if (!cache_hit(kind)) {
+                steps = plan_gates(queue, n, g, Kg, search);
+                cache_store(std::vector<int>(steps.size(), Kg));
+            }
+        } else {
+            steps = plan_gates(queue, n, g, Kg, search);
+        }
         progs = build_all(steps, Kg);
         return;
+    }
+    const bool cacheable = search && mode == PLAN_LEARNED;
+    if (cacheable) {
+        // both tilings' inputs, the block width, the model and the kernels
+        // the predictions are for
+        key = plan_key(queue, n, g, 0, search);
+        const std::vector<uint64_t> k2 = plan_key(queue, n, g, Kg, search);
+        key.insert(key.end(), k2.begin(), k2.end());
+        key.insert(key.begin(), {2, uint64_t(Kg), reinterpret_cast<uintptr_t>(cm), uint64_t(st.jit_enabled()),
+                                 uint64_t(factors), uint64_t(prec)});
+        std::vector<int> kind;
+        if (cache_hit(kind)) {
+            progs.assign(steps.size(), SweepProgram{});
+            for (size_t s = 0; s < steps.size(); ++s)
+                if (steps[s].sweep) progs[s] = build_candidate(queue, steps[s], g, kind[s], search, factors);
+            return;
+        }
     }
     // candidates: tiling A (per-gate tiles) and tiling B (fusion tiles), each
     // sweep of either with one program per block width (0: register phases,
@@ -340,8 +402,13 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     }
     steps = std::move(tl[use]);
     progs.resize(steps.size());
+    std::vector<int> kind(steps.size(), 0);
     for (size_t s = 0; s < steps.size(); ++s)
-        if (steps[s].sweep) progs[s] = std::move(cand[use][s][pick[use][s]]);
+        if (steps[s].sweep) {
+            progs[s] = std::move(cand[use][s][pick[use][s]]);
+            kind[s] = kinds[pick[use][s]];
+        }
+    if (cacheable) cache_store(kind);
 }
 
 std::vector<std::string> plan_jit_sources(const std::vector<PendingGate>& q, int n, Precision prec, int fusion_k,
