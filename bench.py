@@ -179,7 +179,13 @@ def batch_extra(count, seed=2024):
     gen_s = time.perf_counter() - t0
     out = {"circuits": count, "nr_runs": 64, "qubits": [10, 22], "depth": [10, 40], "measurements": [1, 3],
            "program_ops": int(sum(len(p) for _, _, p, _ in wl)), "generate_s": round(gen_s, 2)}
+    from q1tsim_amd.gvs import set_device_cache_limit
     for prec in ("c64", "c128"):
+        # each precision starts from an empty device block cache, as it would
+        # in a process of its own: the blocks the earlier legs released (the
+        # 30- and 32-qubit states) are no use to the batch's small states
+        set_device_cache_limit(1)
+        set_device_cache_limit(0)
         circs = []
         for n, nc, prog, cs in wl:
             c = Circuit(n, nc)
