@@ -48,7 +48,14 @@ public:
     void set_norm(const float* mu, const float* sigma);
     void get_norm(float* mu, float* sigma) const;
     bool trained() const { return trained_; }
-    void mark_trained(bool t) { trained_ = t; }
+    void mark_trained(bool t)
+    {
+        trained_ = t;
+        ++version_;
+    }
+    // changes with every change of parameters, scaling or training state
+    // (plans chosen with the model are kept only while it is unchanged)
+    uint64_t version() const { return version_; }
 
     // y[i] = f(x[i]) for nrows rows of kCmFeat features (host arrays)
     void predict(const float* x, size_t nrows, float* y);
@@ -78,6 +85,7 @@ private:
     double* d_loss_ = nullptr;
     size_t rows_cap_ = 0;
     uint64_t step_ = 0;
+    uint64_t version_ = 0;
     bool trained_ = false;
 
     void ensure_rows(size_t nrows);

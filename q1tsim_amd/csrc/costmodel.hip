@@ -230,6 +230,7 @@ CostModel::~CostModel()
 
 void CostModel::init(uint64_t seed)
 {
+    ++version_;
     std::vector<float> p(kCmParams, 0.0f);
     uint64_t s = seed;
     auto uni = [&]() {   // SplitMix64 -> U[0,1)
@@ -261,6 +262,7 @@ void CostModel::reset_optimizer()
 
 void CostModel::set_params(const float* p)
 {
+    ++version_;
     CM_HIP_OK(hipMemcpyAsync(d_params_, p, kCmParams * sizeof(float), hipMemcpyHostToDevice, stream_));
     CM_HIP_OK(hipStreamSynchronize(stream_));
 }
@@ -273,6 +275,7 @@ void CostModel::get_params(float* p) const
 
 void CostModel::set_norm(const float* mu, const float* sigma)
 {
+    ++version_;
     float h[2 * kCmFeat];
     for (int f = 0; f < kCmFeat; ++f) {
         if (!(sigma[f] > 0.0f)) throw Error(ERR_INVALID_ARGUMENT, "Feature scale must be positive");
@@ -384,6 +387,7 @@ static void adam_launch(hipStream_t st, float* P, float* m, float* v, const floa
 
 double CostModel::adam_step(const float* x, const float* t, size_t nrows, const AdamConfig& cfg)
 {
+    ++version_;
     if (!nrows) throw Error(ERR_INVALID_ARGUMENT, "Empty batch");
     ensure_rows(nrows);
     CM_HIP_OK(hipMemcpyAsync(d_x_, x, nrows * kCmFeat * sizeof(float), hipMemcpyHostToDevice, stream_));
@@ -398,6 +402,7 @@ double CostModel::train(const float* x, const float* t, size_t nrows, int epochs
                         const AdamConfig& cfg, uint64_t seed)
 {
     if (!nrows || !batch) throw Error(ERR_INVALID_ARGUMENT, "Empty training set or batch");
+    ++version_;
     batch = std::min(batch, nrows);
     ensure_rows(nrows);
     ensure_partials((batch + kCmRowsPerBlock - 1) / kCmRowsPerBlock);

@@ -271,13 +271,15 @@ static void choose_plan(GpuVectorState& st, const std::vector<PendingGate>& queu
     }
     const bool cacheable = search && mode == PLAN_LEARNED;
     if (cacheable) {
-        // both tilings' inputs, the block width, the model and the kernels
-        // the predictions are for
+        // both tilings' inputs, the block width, the model, the kernels the
+        // predictions are for and the bytes swept (single gates are costed
+        // by them)
         key = plan_key(queue, n, g, 0, search);
         const std::vector<uint64_t> k2 = plan_key(queue, n, g, Kg, search);
         key.insert(key.end(), k2.begin(), k2.end());
-        key.insert(key.begin(), {2, uint64_t(Kg), reinterpret_cast<uintptr_t>(cm), uint64_t(st.jit_enabled()),
-                                 uint64_t(factors), uint64_t(prec)});
+        key.insert(key.begin(), {2, uint64_t(Kg), reinterpret_cast<uintptr_t>(cm), cm->version(),
+                                 uint64_t(st.jit_enabled()), uint64_t(factors), uint64_t(prec),
+                                 uint64_t(state_bytes)});
         std::vector<int> kind;
         if (cache_hit(kind)) {
             progs.assign(steps.size(), SweepProgram{});
